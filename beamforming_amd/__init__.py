@@ -1,0 +1,210 @@
+"""ctypes binding of libbeamform_amd.so (the C ABI in include/beamform.h).
+
+This is the Python face of the library, used by the tests and the benchmark.
+It exposes the same operation as the reference's entry point
+``form_beams(J, E, A, G)`` (reference cpu.cxx:15-16): 4-bit complex voltages E
+and weights A in ucomplex4 byte encoding, float32 gains G, 4-bit complex beams
+J. Device buffers are passed as raw pointers (for example
+``torch.Tensor.data_ptr()``), streams as ``hipStream_t`` integers (for example
+``torch.cuda.current_stream().cuda_stream``).
+
+The library must be built (``make -C beamforming_amd``); importing this
+module raises if it is missing. There is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "BF_J_BFPT", "BF_J_BFTP", "BeamformError", "Shape", "Plan", "lib", "library_path",
+    "form_beams", "adler32", "adler32_device", "fill_uniform_device", "version",
+]
+
+BF_J_BFPT = 0  # J[b][f][p][t] (reference Jlinear, arraysizes.hxx:91-101)
+BF_J_BFTP = 1  # J[b][f][t][p] (reference J2linear, arraysizes.hxx:103-112)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(_HERE, "libbeamform_amd.so")
+
+
+class BeamformError(RuntimeError):
+    """A non-zero status returned by the library."""
+
+    def __init__(self, status, message):
+        super().__init__(message)
+        self.status = status
+
+
+class Shape(ctypes.Structure):
+    """bf_shape: T, F, D, P, B and the J layout."""
+
+    _fields_ = [
+        ("ntimes", ctypes.c_int64),
+        ("nfrequencies", ctypes.c_int64),
+        ("ndishes", ctypes.c_int64),
+        ("npolarizations", ctypes.c_int64),
+        ("nbeams", ctypes.c_int64),
+        ("j_layout", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+    @classmethod
+    def make(cls, ntimes, nfrequencies, ndishes, nbeams, npolarizations=2, j_layout=BF_J_BFPT):
+        return cls(ntimes, nfrequencies, ndishes, npolarizations, nbeams, j_layout, 0)
+
+    @property
+    def e_bytes(self):
+        return self.ntimes * self.nfrequencies * self.ndishes * self.npolarizations
+
+    @property
+    def a_bytes(self):
+        return self.nfrequencies * self.nbeams * self.ndishes
+
+    @property
+    def g_count(self):
+        return self.nfrequencies * self.nbeams
+
+    @property
+    def j_bytes(self):
+        return self.nbeams * self.nfrequencies * self.npolarizations * self.ntimes
+
+    def as_tuple(self):
+        return (self.ntimes, self.nfrequencies, self.ndishes, self.npolarizations, self.nbeams, self.j_layout)
+
+
+def _load():
+    if not os.path.exists(library_path):
+        raise ImportError(f"{library_path} is not built; run `make -C {_HERE}` (hipcc, gfx950)")
+    lib = ctypes.CDLL(library_path)
+    vp, u8p, fp = ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p
+    sig = {
+        "bf_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+        "bf_version": (ctypes.c_char_p, []),
+        "bf_last_error": (ctypes.c_char_p, []),
+        "bf_plan_create": (ctypes.c_int, [ctypes.POINTER(Shape), u8p, fp, ctypes.POINTER(vp)]),
+        "bf_plan_create_device": (ctypes.c_int, [ctypes.POINTER(Shape), u8p, fp, vp, ctypes.POINTER(vp)]),
+        "bf_plan_update_weights": (ctypes.c_int, [vp, u8p, fp]),
+        "bf_plan_destroy": (ctypes.c_int, [vp]),
+        "bf_plan_shape": (ctypes.c_int, [vp, ctypes.POINTER(Shape)]),
+        "bf_plan_describe": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_size_t]),
+        "bf_form_beams_device": (ctypes.c_int, [vp, u8p, u8p, vp]),
+        "bf_form_beams_host": (ctypes.c_int, [vp, u8p, u8p]),
+        "bf_form_beams": (ctypes.c_int, [u8p, u8p, u8p, fp, ctypes.POINTER(Shape)]),
+        "bf_adler32_device": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32), vp]),
+        "bf_adler32": (ctypes.c_uint32, [u8p, ctypes.c_size_t]),
+        "bf_fill_uniform_device": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _check(status):
+    if status != 0:
+        msg = lib.bf_last_error().decode() or lib.bf_status_string(status).decode()
+        raise BeamformError(status, f"{lib.bf_status_string(status).decode()}: {msg}")
+
+
+def _host_ptr(arr, dtype):
+    a = np.ascontiguousarray(arr, dtype=dtype)
+    return a, a.ctypes.data_as(ctypes.c_void_p)
+
+
+def version():
+    return lib.bf_version().decode()
+
+
+class Plan:
+    """A form-beams plan: shape + resident weights/gains + kernel choice."""
+
+    def __init__(self, shape, A=None, G=None, *, A_dev=None, G_dev=None, stream=0):
+        self.shape = shape
+        self._h = ctypes.c_void_p()
+        if A_dev is not None:
+            _check(lib.bf_plan_create_device(ctypes.byref(shape), ctypes.c_void_p(A_dev), ctypes.c_void_p(G_dev),
+                                             ctypes.c_void_p(stream), ctypes.byref(self._h)))
+        else:
+            A_arr, A_p = _host_ptr(A, np.uint8)
+            G_arr, G_p = _host_ptr(G, np.float32)
+            if A_arr.size != shape.a_bytes or G_arr.size != shape.g_count:
+                raise ValueError("A or G size does not match the shape")
+            _check(lib.bf_plan_create(ctypes.byref(shape), A_p, G_p, ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib.bf_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def describe(self):
+        buf = ctypes.create_string_buffer(512)
+        _check(lib.bf_plan_describe(self._h, buf, 512))
+        return buf.value.decode()
+
+    def update_weights(self, A, G):
+        A_arr, A_p = _host_ptr(A, np.uint8)
+        G_arr, G_p = _host_ptr(G, np.float32)
+        _check(lib.bf_plan_update_weights(self._h, A_p, G_p))
+
+    def form_beams_device(self, J_ptr, E_ptr, stream=0):
+        """Asynchronous launch on device buffers (raw pointers)."""
+        _check(lib.bf_form_beams_device(self._h, ctypes.c_void_p(J_ptr), ctypes.c_void_p(E_ptr),
+                                        ctypes.c_void_p(stream)))
+
+    def form_beams_host(self, E):
+        """Synchronous: host E (numpy uint8) -> host J (numpy uint8)."""
+        E_arr, E_p = _host_ptr(E, np.uint8)
+        if E_arr.size != self.shape.e_bytes:
+            raise ValueError("E size does not match the shape")
+        J = np.empty(self.shape.j_bytes, dtype=np.uint8)
+        _check(lib.bf_form_beams_host(self._h, J.ctypes.data_as(ctypes.c_void_p), E_p))
+        return J
+
+
+def form_beams(E, A, G, shape):
+    """One-shot host call: returns J (numpy uint8) for host E, A, G."""
+    E_arr, E_p = _host_ptr(E, np.uint8)
+    A_arr, A_p = _host_ptr(A, np.uint8)
+    G_arr, G_p = _host_ptr(G, np.float32)
+    J = np.empty(shape.j_bytes, dtype=np.uint8)
+    _check(lib.bf_form_beams(J.ctypes.data_as(ctypes.c_void_p), E_p, A_p, G_p, ctypes.byref(shape)))
+    return J
+
+
+def adler32(data):
+    """Adler-32 of host bytes (reference adler32.c:7-20)."""
+    arr, p = _host_ptr(data, np.uint8)
+    return int(lib.bf_adler32(p, arr.size))
+
+
+def adler32_device(ptr, nbytes, stream=0):
+    """Adler-32 of device bytes, computed on the device."""
+    out = ctypes.c_uint32()
+    _check(lib.bf_adler32_device(ctypes.c_void_p(ptr), nbytes, ctypes.byref(out), ctypes.c_void_p(stream)))
+    return int(out.value)
+
+
+def fill_uniform_device(ptr, nbytes, seed, offset=0, stream=0):
+    """Seeded uniform ucomplex4 bytes into device memory (same stream as host oracle)."""
+    _check(lib.bf_fill_uniform_device(ctypes.c_void_p(ptr), nbytes, seed, offset, ctypes.c_void_p(stream)))

@@ -1,0 +1,413 @@
+// bf_api.hip -- the C ABI of libbeamform_amd.so (declared in include/beamform.h).
+//
+// A plan holds everything that depends only on the shape and the weights:
+// the weight operands expanded on the device, the per-beam offsets, the
+// padded gains and the kernel configuration. form_beams calls are then a
+// single launch on the caller's stream.
+#include "beamform.h"
+#include "bf_plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace bf {
+
+thread_local std::string g_last_error;
+
+int fail(int status, const std::string &msg) {
+  g_last_error = msg;
+  return status;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? BF_ERR_OUT_OF_MEMORY : BF_ERR_DEVICE;
+}
+
+#define BF_HIP(call)                                                                                                   \
+  do {                                                                                                                 \
+    const hipError_t e_ = (call);                                                                                      \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);                                                                  \
+  } while (0)
+
+int check_shape(const bf_shape *s) {
+  if (!s) return fail(BF_ERR_INVALID_ARGUMENT, "shape is NULL");
+  if (s->ntimes <= 0 || s->nfrequencies <= 0 || s->ndishes <= 0 || s->npolarizations <= 0 || s->nbeams <= 0)
+    return fail(BF_ERR_INVALID_ARGUMENT, "all shape extents must be positive");
+  if (s->j_layout != BF_J_BFPT && s->j_layout != BF_J_BFTP) return fail(BF_ERR_INVALID_ARGUMENT, "bad j_layout");
+  if (s->reserved != 0) return fail(BF_ERR_INVALID_ARGUMENT, "reserved field must be 0");
+  if (s->nfrequencies > (1 << 20) || s->nbeams > (1 << 20) || s->ndishes > (1 << 20) || s->npolarizations > 64)
+    return fail(BF_ERR_UNSUPPORTED, "shape extent too large");
+  // int32 accumulation is exact for |raw| <= 128 D < 2^31 and the int -> float
+  // conversion of raw is exact below 2^24 (as in the reference).
+  if (s->ndishes > 65536) return fail(BF_ERR_UNSUPPORTED, "ndishes > 65536");
+  return BF_OK;
+}
+
+// Largest |G| for which |G * raw| < 2^31 holds for every reachable raw
+// (|raw| <= 128 D), with a factor-2 margin for the float rounding.
+static bool gains_fast_safe(const float *G, int64_t n, int64_t D) {
+  const double limit = std::ldexp(1.0, 30) / (128.0 * (double)D);
+  for (int64_t i = 0; i < n; ++i)
+    if (!std::isfinite(G[i]) || std::fabs((double)G[i]) >= limit) return false;
+  return true;
+}
+
+static MfmaKernel lookup_mfma(int nt, int ks) {
+  switch (ks) {
+  case 1: return mfma_kernel_ks1(nt);
+  case 2: return mfma_kernel_ks2(nt);
+  case 4: return mfma_kernel_ks4(nt);
+  case 8: return mfma_kernel_ks8(nt);
+  case 16: return mfma_kernel_ks16(nt);
+  case 32: return mfma_kernel_ks32(nt);
+  default: return nullptr;
+  }
+}
+
+// Chooses the MFMA configuration for a shape. Returns false if the shape is
+// outside the MFMA kernel's range.
+bool choose_mfma_config(const bf_shape &s, int num_cus, MfmaConfig *cfg) {
+  if (s.npolarizations != 2 || s.ndishes % 8 != 0 || s.ndishes > 1024) return false;
+  const int kneed = (int)((s.ndishes + kDishesPerStep - 1) / kDishesPerStep);
+  int ks = 1;
+  while (ks < kneed) ks *= 2;
+  const int nt_tot = (int)((s.nbeams + 15) / 16);
+  static const int cand[] = {1, 2, 3, 4, 6};
+  const int max_nt = ks >= 32 ? 4 : 6; // LDS: ks * nt KiB; registers cap nt at 6
+  // Fewest padded beam tiles, then fewest beam groups (E passes).
+  int best_nt = 0, best_nbg = 0;
+  for (int nt : cand) {
+    if (nt > max_nt) continue;
+    const int nbg = (nt_tot + nt - 1) / nt;
+    const int padded = nbg * nt;
+    if (best_nt == 0 || padded < best_nt * best_nbg || (padded == best_nt * best_nbg && nbg < best_nbg)) {
+      best_nt = nt;
+      best_nbg = nbg;
+    }
+  }
+  cfg->ks = ks;
+  cfg->nt = best_nt;
+  cfg->nbg = best_nbg;
+  cfg->ntp = best_nt * best_nbg;
+  const int64_t ntiles = (s.ntimes + kTileT - 1) / kTileT;
+  const int64_t cols = s.nfrequencies * best_nbg;
+  const int64_t target = std::max<int64_t>(num_cus, 1);
+  const int64_t chunks0 = std::max<int64_t>(1, (target + cols - 1) / cols);
+  int64_t tpc = (ntiles + chunks0 - 1) / chunks0;
+  tpc = (tpc + kWaves - 1) / kWaves * kWaves;
+  const int64_t nchunks = (ntiles + tpc - 1) / tpc;
+  if (cols * nchunks > (1ll << 30)) return false;
+  cfg->tiles_per_chunk = (int)tpc;
+  cfg->nchunks = (int)nchunks;
+  cfg->nwork = (int)(cols * nchunks);
+  cfg->per_xcd = (cfg->nwork + 7) / 8;
+  cfg->grid = 8 * cfg->per_xcd;
+  cfg->lds_bytes = (size_t)ks * best_nt * 64 * sizeof(v4i);
+  cfg->kernel = lookup_mfma(best_nt, ks);
+  return cfg->kernel != nullptr;
+}
+
+static int num_cus_of_current_device(int *dev, int *ncu) {
+  int d = 0;
+  BF_HIP(hipGetDevice(&d));
+  int n = 0;
+  BF_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, d));
+  *dev = d;
+  *ncu = n;
+  return BF_OK;
+}
+
+// (Re)builds the device-side weight data of a plan from device A and G.
+static int prepare_weights(bf_plan *p, hipStream_t st) {
+  const bf_shape &s = p->shape;
+  const int F = (int)s.nfrequencies, B = (int)s.nbeams, D = (int)s.ndishes;
+  if (p->use_mfma) {
+    const MfmaConfig &c = p->cfg;
+    const int64_t nw = (int64_t)F * c.ks * c.ntp * 64;
+    const int blocks = (int)std::min<int64_t>((nw + 255) / 256, 65536);
+    hipLaunchKernelGGL(prep_weights_kernel, dim3(blocks), dim3(256), 0, st, p->A_dev, p->W, F, B, D, c.ks, c.ntp);
+    BF_HIP(hipGetLastError());
+    const int BP = c.ntp * 16;
+    const int64_t nc = (int64_t)F * BP;
+    const int blocks2 = (int)std::min<int64_t>((nc + 255) / 256, 65536);
+    hipLaunchKernelGGL(prep_corr_kernel, dim3(blocks2), dim3(256), 0, st, p->A_dev, p->G_dev, p->corr, p->Gp, F, B, D,
+                       BP);
+    BF_HIP(hipGetLastError());
+  }
+  BF_HIP(hipStreamSynchronize(st));
+  return BF_OK;
+}
+
+static void free_plan(bf_plan *p) {
+  if (!p) return;
+  if (p->W) (void)hipFree(p->W);
+  if (p->corr) (void)hipFree(p->corr);
+  if (p->Gp) (void)hipFree(p->Gp);
+  if (p->A_dev) (void)hipFree(p->A_dev);
+  if (p->G_dev) (void)hipFree(p->G_dev);
+  delete p;
+}
+
+// Creates a plan. A and G are host pointers (host_inputs) or device pointers.
+static int plan_create(const bf_shape *shape, const uint8_t *A, const float *G, bool host_inputs, hipStream_t st,
+                       bf_plan **out) {
+  if (!out) return fail(BF_ERR_INVALID_ARGUMENT, "plan output pointer is NULL");
+  *out = nullptr;
+  if (int rc = check_shape(shape)) return rc;
+  if (!A || !G) return fail(BF_ERR_INVALID_ARGUMENT, "A or G is NULL");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(BF_ERR_NO_DEVICE, "no HIP device visible");
+
+  bf_plan *p = new bf_plan();
+  p->shape = *shape;
+  int rc = BF_OK;
+  auto bail = [&](int code) {
+    free_plan(p);
+    return code;
+  };
+  if ((rc = num_cus_of_current_device(&p->device, &p->num_cus))) return bail(rc);
+  const bf_shape &s = p->shape;
+  const int64_t nA = s.nfrequencies * s.nbeams * s.ndishes;
+  const int64_t nG = s.nfrequencies * s.nbeams;
+
+  std::vector<float> Gh;
+  const float *Gcheck = G;
+  if (!host_inputs) {
+    Gh.resize(nG);
+    hipError_t e = hipMemcpyAsync(Gh.data(), G, nG * sizeof(float), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return bail(hip_fail(e, "copy G for range check"));
+    Gcheck = Gh.data();
+  }
+  p->fast_gains = gains_fast_safe(Gcheck, nG, s.ndishes);
+  p->use_mfma = p->fast_gains && choose_mfma_config(s, p->num_cus, &p->cfg);
+
+  hipError_t e;
+  if ((e = hipMalloc(&p->A_dev, nA)) != hipSuccess) return bail(hip_fail(e, "hipMalloc A"));
+  if ((e = hipMalloc(&p->G_dev, nG * sizeof(float))) != hipSuccess) return bail(hip_fail(e, "hipMalloc G"));
+  const hipMemcpyKind kind = host_inputs ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  if ((e = hipMemcpyAsync(p->A_dev, A, nA, kind, st)) != hipSuccess) return bail(hip_fail(e, "copy A"));
+  if ((e = hipMemcpyAsync(p->G_dev, G, nG * sizeof(float), kind, st)) != hipSuccess) return bail(hip_fail(e, "copy G"));
+  if (p->use_mfma) {
+    const MfmaConfig &c = p->cfg;
+    const size_t wbytes = (size_t)s.nfrequencies * c.ks * c.ntp * 64 * sizeof(v4i);
+    const size_t BP = (size_t)c.ntp * 16;
+    if ((e = hipMalloc(&p->W, wbytes)) != hipSuccess) return bail(hip_fail(e, "hipMalloc W"));
+    if ((e = hipMalloc(&p->corr, s.nfrequencies * BP * 2 * sizeof(int))) != hipSuccess)
+      return bail(hip_fail(e, "hipMalloc corr"));
+    if ((e = hipMalloc(&p->Gp, s.nfrequencies * BP * sizeof(float))) != hipSuccess)
+      return bail(hip_fail(e, "hipMalloc Gp"));
+    if ((e = hipFuncSetAttribute(reinterpret_cast<const void *>(c.kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)c.lds_bytes)) != hipSuccess)
+      return bail(hip_fail(e, "hipFuncSetAttribute(MaxDynamicSharedMemorySize)"));
+  }
+  if ((rc = prepare_weights(p, st))) return bail(rc);
+  *out = p;
+  return BF_OK;
+}
+
+int launch_form_beams(const bf_plan *p, uint8_t *J, const uint8_t *E, hipStream_t st) {
+  const bf_shape &s = p->shape;
+  // The MFMA kernel reads E in 16-byte vectors and writes J in 4/8-byte words.
+  const bool aligned = ((uintptr_t)E & 15u) == 0 && ((uintptr_t)J & 7u) == 0;
+  if (p->use_mfma && aligned) {
+    const MfmaConfig &c = p->cfg;
+    MfmaParams a;
+    a.E = E;
+    a.J = J;
+    a.W = p->W;
+    a.corr = p->corr;
+    a.G = p->Gp;
+    a.T = s.ntimes;
+    a.F = (int32_t)s.nfrequencies;
+    a.D = (int32_t)s.ndishes;
+    a.B = (int32_t)s.nbeams;
+    a.NTP = c.ntp;
+    a.nbg = c.nbg;
+    a.nchunks = c.nchunks;
+    a.tiles_per_chunk = c.tiles_per_chunk;
+    a.nwork = c.nwork;
+    a.per_xcd = c.per_xcd;
+    a.jlayout = s.j_layout;
+    void *args[] = {&a};
+    BF_HIP(hipLaunchKernel(reinterpret_cast<const void *>(c.kernel), dim3(c.grid), dim3(kThreads), args, c.lds_bytes,
+                           st));
+  } else {
+    const int64_t total = s.nbeams * s.nfrequencies * s.npolarizations * s.ntimes;
+    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 1 << 20);
+    hipLaunchKernelGGL(form_beams_generic_kernel, dim3(blocks), dim3(256), 0, st, E, J, p->A_dev, p->G_dev, s.ntimes,
+                       (int)s.nfrequencies, (int)s.ndishes, (int)s.npolarizations, (int)s.nbeams, s.j_layout);
+    BF_HIP(hipGetLastError());
+  }
+  return BF_OK;
+}
+
+} // namespace bf
+
+using namespace bf;
+
+extern "C" {
+
+const char *bf_status_string(int status) {
+  switch (status) {
+  case BF_OK: return "ok";
+  case BF_ERR_INVALID_ARGUMENT: return "invalid argument";
+  case BF_ERR_UNSUPPORTED: return "unsupported shape";
+  case BF_ERR_DEVICE: return "device error";
+  case BF_ERR_OUT_OF_MEMORY: return "out of memory";
+  case BF_ERR_OVER_BUDGET: return "over HBM budget";
+  case BF_ERR_NOT_READY: return "not ready";
+  case BF_ERR_CLOSED: return "closed";
+  case BF_ERR_NO_DEVICE: return "no device";
+  default: return "unknown status";
+  }
+}
+
+const char *bf_version(void) { return "beamforming_amd 0.1 (gfx950, int8 MFMA)"; }
+
+const char *bf_last_error(void) { return g_last_error.c_str(); }
+
+int bf_plan_create(const bf_shape *shape, const uint8_t *A, const float *G, bf_plan **plan) {
+  return plan_create(shape, A, G, true, nullptr, plan);
+}
+
+int bf_plan_create_device(const bf_shape *shape, const uint8_t *A_dev, const float *G_dev, void *stream,
+                          bf_plan **plan) {
+  return plan_create(shape, A_dev, G_dev, false, reinterpret_cast<hipStream_t>(stream), plan);
+}
+
+int bf_plan_update_weights(bf_plan *p, const uint8_t *A, const float *G) {
+  if (!p || !A || !G) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  const bf_shape s = p->shape;
+  const int64_t nG = s.nfrequencies * s.nbeams;
+  const bool fast = gains_fast_safe(G, nG, s.ndishes);
+  if (fast != p->fast_gains) {
+    // The kernel family changes; rebuild the plan in place.
+    bf_plan *q = nullptr;
+    if (int rc = plan_create(&s, A, G, true, nullptr, &q)) return rc;
+    std::swap(*p, *q);
+    free_plan(q);
+    return BF_OK;
+  }
+  BF_HIP(hipMemcpy(p->A_dev, A, s.nfrequencies * s.nbeams * s.ndishes, hipMemcpyHostToDevice));
+  BF_HIP(hipMemcpy(p->G_dev, G, nG * sizeof(float), hipMemcpyHostToDevice));
+  return prepare_weights(p, nullptr);
+}
+
+int bf_plan_destroy(bf_plan *p) {
+  free_plan(p);
+  return BF_OK;
+}
+
+int bf_plan_shape(const bf_plan *p, bf_shape *shape) {
+  if (!p || !shape) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  *shape = p->shape;
+  return BF_OK;
+}
+
+int bf_plan_describe(const bf_plan *p, char *buf, size_t len) {
+  if (!p || !buf || len == 0) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (p->use_mfma) {
+    const MfmaConfig &c = p->cfg;
+    std::snprintf(buf, len,
+                  "mfma_i32_16x16x64_i8 nt=%d ks=%d nbg=%d tile_t=%d waves=%d tiles_per_chunk=%d nchunks=%d grid=%d "
+                  "lds=%zu",
+                  c.nt, c.ks, c.nbg, kTileT, kWaves, c.tiles_per_chunk, c.nchunks, c.grid, c.lds_bytes);
+  } else {
+    std::snprintf(buf, len, "generic (%s)", p->fast_gains ? "shape outside MFMA range" : "gain outside fast range");
+  }
+  return BF_OK;
+}
+
+int bf_form_beams_device(const bf_plan *p, uint8_t *J, const uint8_t *E, void *stream) {
+  if (!p || !J || !E) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  return launch_form_beams(p, J, E, reinterpret_cast<hipStream_t>(stream));
+}
+
+int bf_form_beams_host(const bf_plan *p, uint8_t *J, const uint8_t *E) {
+  if (!p || !J || !E) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  const bf_shape &s = p->shape;
+  const size_t nE = (size_t)(s.ntimes * s.nfrequencies * s.ndishes * s.npolarizations);
+  const size_t nJ = (size_t)(s.nbeams * s.nfrequencies * s.npolarizations * s.ntimes);
+  uint8_t *dE = nullptr, *dJ = nullptr;
+  hipError_t e = hipMalloc(&dE, nE);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc E");
+  e = hipMalloc(&dJ, nJ);
+  int rc = BF_OK;
+  if (e != hipSuccess) {
+    rc = hip_fail(e, "hipMalloc J");
+  } else if ((e = hipMemcpy(dE, E, nE, hipMemcpyHostToDevice)) != hipSuccess) {
+    rc = hip_fail(e, "copy E");
+  } else if ((rc = launch_form_beams(p, dJ, dE, nullptr)) == BF_OK) {
+    if ((e = hipMemcpy(J, dJ, nJ, hipMemcpyDeviceToHost)) != hipSuccess) rc = hip_fail(e, "copy J");
+  }
+  (void)hipFree(dE);
+  if (dJ) (void)hipFree(dJ);
+  return rc;
+}
+
+int bf_form_beams(uint8_t *J, const uint8_t *E, const uint8_t *A, const float *G, const bf_shape *shape) {
+  bf_plan *p = nullptr;
+  if (int rc = bf_plan_create(shape, A, G, &p)) return rc;
+  const int rc = bf_form_beams_host(p, J, E);
+  bf_plan_destroy(p);
+  return rc;
+}
+
+uint32_t bf_adler32(const uint8_t *data, size_t len) {
+  // Same recurrence as adler32.c:7-20, with the modulo deferred over blocks
+  // of 5552 bytes (the largest count for which b cannot overflow 32 bits).
+  const uint32_t mod = 65521u;
+  uint32_t a = 1, b = 0;
+  while (len > 0) {
+    const size_t n = len < 5552 ? len : 5552;
+    for (size_t i = 0; i < n; ++i) {
+      a += data[i];
+      b += a;
+    }
+    a %= mod;
+    b %= mod;
+    data += n;
+    len -= n;
+  }
+  return (b << 16) | a;
+}
+
+int bf_adler32_device(const uint8_t *data, size_t len, uint32_t *out, void *stream) {
+  if (!data || !out) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  unsigned long long *acc = nullptr;
+  BF_HIP(hipMalloc(&acc, 2 * sizeof(unsigned long long)));
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, acc, 2);
+  const uint64_t nvec = len / 16;
+  const int blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>((nvec + 255) / 256, 4096));
+  hipLaunchKernelGGL(adler32_partial_kernel, dim3(blocks), dim3(256), 0, st, data, (uint64_t)len, acc);
+  unsigned long long h[2] = {0, 0};
+  hipError_t e = hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(acc);
+  if (e != hipSuccess) return hip_fail(e, "adler32 device reduction");
+  const uint64_t mod = 65521u;
+  const uint64_t a = (1 + h[0]) % mod;
+  const uint64_t b = ((uint64_t)(len % mod) + h[1]) % mod;
+  *out = (uint32_t)((b << 16) | a);
+  return BF_OK;
+}
+
+int bf_fill_uniform_device(uint8_t *buf, size_t n, uint64_t seed, uint64_t offset, void *stream) {
+  if (!buf && n) return fail(BF_ERR_INVALID_ARGUMENT, "NULL buffer");
+  if (n == 0) return BF_OK;
+  const uint64_t nvec = n / 16;
+  const int blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>((nvec + 255) / 256, 8192));
+  hipLaunchKernelGGL(fill_uniform_kernel, dim3(blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), buf,
+                     (uint64_t)n, seed, offset);
+  BF_HIP(hipGetLastError());
+  return BF_OK;
+}
+
+} // extern "C"

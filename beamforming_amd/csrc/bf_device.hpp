@@ -1,0 +1,184 @@
+// bf_device.hpp -- per-lane arithmetic shared by the gfx950 kernels and the
+// host-side emulator of their data path (bf_emu.cpp).
+//
+// Everything here is integer/byte work except the requantisation, which
+// reproduces the reference's float semantics bit for bit (cpu.cxx:37-39).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#define BF_HD __host__ __device__ __forceinline__
+
+namespace bf {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// ucomplex4 codec (icomplex4.hxx:33-37): offset binary, real part in the high
+// nibble, each part stored as value + 8.
+BF_HD int dec_re(uint32_t x) { return (int)((int8_t)(uint8_t)(x ^ 0x88u)) >> 4; }
+BF_HD int dec_im(uint32_t x) { return (int)((int8_t)(uint8_t)((x ^ 0x88u) << 4)) >> 4; }
+BF_HD uint32_t enc(int re, int im) { return ((((uint32_t)re & 0xfu) << 4) | ((uint32_t)im & 0xfu)) ^ 0x88u; }
+
+// ---------------------------------------------------------------------------
+// GEMM formulation.
+//
+// For one frequency, with E-rows = time samples and columns = beams:
+//   raw_re[t,b] = sum_d Ar Er - Ai Ei,   raw_im[t,b] = sum_d Ar Ei + Ai Er.
+// The weights are held once as int8 pairs W = (Ar, Ai) per dish. The E side
+// is fed to the int8 MFMA as the raw nibbles s = value + 8 (0..15), in two
+// variants that share W:
+//   v1 = (s_re, 15 - s_im)  ->  W.v1 = raw_re + 8 sum Ar + 7 sum Ai
+//   v2 = (s_im, s_re)       ->  W.v2 = raw_im + 8 sum Ar + 8 sum Ai
+// The constant offsets depend only on (f, b); the accumulators start at their
+// negation, so they end at exactly raw_re / raw_im (exact int32 arithmetic).
+//
+// MFMA v_mfma_i32_16x16x64_i8 operand: each lane holds 16 int8 of one row
+// (E) or column (W). Per lane and K step the 16 k-values cover 8 dishes as
+// four dwords: [re d0-3 | im d0-3 | re d4-7 | im d4-7] on the weight side
+// and the matching v1 / v2 dwords on the E side.
+
+// Byte permute: result byte i = byte sel_i of the 8-byte value (hi:lo),
+// selectors 0-3 address lo, 4-7 address hi (v_perm_b32).
+BF_HD uint32_t perm_bytes(uint32_t hi, uint32_t lo, uint32_t sel) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_perm(hi, lo, sel);
+#else
+  const uint64_t v = ((uint64_t)hi << 32) | lo;
+  uint32_t r = 0;
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t s = (sel >> (8 * i)) & 0xffu;
+    r |= (uint32_t)((v >> (8 * s)) & 0xffu) << (8 * i);
+  }
+  return r;
+#endif
+}
+
+// Splits 16 packed E bytes (8 dishes x 2 polarizations: byte 2k+p = dish k,
+// polarization p) into the MFMA operands of both polarizations and both
+// variants: op[2p+0] = v1 of polarization p, op[2p+1] = v2.
+BF_HD void unpack_e(const v4i e, v4i op[4]) {
+  const uint32_t x0 = (uint32_t)e.x, x1 = (uint32_t)e.y, x2 = (uint32_t)e.z, x3 = (uint32_t)e.w;
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const uint32_t sel = p == 0 ? 0x06040200u : 0x07050301u;
+    const uint32_t qa = perm_bytes(x1, x0, sel); // dishes 0-3 of polarization p
+    const uint32_t qb = perm_bytes(x3, x2, sel); // dishes 4-7
+    const uint32_t lo_a = qa & 0x0f0f0f0fu, hi_a = (qa >> 4) & 0x0f0f0f0fu;
+    const uint32_t lo_b = qb & 0x0f0f0f0fu, hi_b = (qb >> 4) & 0x0f0f0f0fu;
+    op[2 * p + 0] = v4i{(int)hi_a, (int)(lo_a ^ 0x0f0f0f0fu), (int)hi_b, (int)(lo_b ^ 0x0f0f0f0fu)};
+    op[2 * p + 1] = v4i{(int)lo_a, (int)hi_a, (int)lo_b, (int)hi_b};
+  }
+}
+
+// Weight operand for one lane: 8 packed A bytes (dishes d0..d0+7, zero-padded
+// by the caller) -> [Ar d0-3 | Ai d0-3 | Ar d4-7 | Ai d4-7] as int8.
+BF_HD v4i pack_weights(const uint8_t a[8]) {
+  uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t x = a[4 * q + i];
+      w[2 * q + 0] |= ((uint32_t)dec_re(x) & 0xffu) << (8 * i);
+      w[2 * q + 1] |= ((uint32_t)dec_im(x) & 0xffu) << (8 * i);
+    }
+  return v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+}
+
+// ---------------------------------------------------------------------------
+// Requantisation, cpu.cxx:37-39: max(-7, min(7, int(lrint(G * float(raw))))).
+//
+// Fast form, valid when G is finite and |G * raw| < 2^31 for every raw the
+// shape can produce (the plan checks this): clamping before rounding gives the
+// same value because rint is monotonic and fixes +-7.
+BF_HD float requant_fast(float g, int raw) {
+  const float y = g * (float)raw;
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_rintf(__builtin_amdgcn_fmed3f(y, -7.0f, 7.0f));
+#else
+  return rintf(fminf(fmaxf(y, -7.0f), 7.0f));
+#endif
+}
+
+// Exact form for any G: lrintf on x86-64 yields INT64_MIN for NaN and
+// |y| >= 2^63, and the long -> int conversion keeps the low 32 bits.
+BF_HD float requant_exact(float g, int raw) {
+  const float y = g * (float)raw;
+  const float ay = fabsf(y);
+  int q;
+  if (ay < 2147483648.0f)
+    q = (int)rintf(y);
+  else if (!(ay < 9223372036854775808.0f))
+    q = 0;
+  else
+    q = (int)(uint32_t)(uint64_t)(int64_t)y; // y is integral here
+  q = q < -7 ? -7 : (q > 7 ? 7 : q);
+  return (float)q;
+}
+
+// Encoded J byte as a float: enc(qr, qi) = 16 (qr + 8) + (qi + 8) for
+// qr, qi in [-7, 7]; every step is exact in float.
+BF_HD float jbyte_f(float qr, float qi) { return fmaf(qr, 16.0f, qi) + 136.0f; }
+
+// Inserts the byte value v (an exact integer 0..255 as float) at byte pos.
+BF_HD uint32_t put_byte(float v, int pos, uint32_t word) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_cvt_pk_u8_f32(v, pos, word);
+#else
+  return (word & ~(0xffu << (8 * pos))) | ((uint32_t)(int)v << (8 * pos));
+#endif
+}
+
+// Offsets that the E-side nibble encoding adds to the two accumulators.
+BF_HD void corrections(int sum_ar, int sum_ai, int *corr_re, int *corr_im) {
+  *corr_re = 8 * sum_ar + 7 * sum_ai;
+  *corr_im = 8 * sum_ar + 8 * sum_ai;
+}
+
+// ---------------------------------------------------------------------------
+// Work decomposition of the MFMA kernel.
+constexpr int kWaves = 8;            // waves per workgroup (2 per SIMD)
+constexpr int kThreads = 64 * kWaves;
+constexpr int kTileT = 16;           // time samples per wave tile (MFMA M)
+constexpr int kDishesPerStep = 32;   // dishes per K step (MFMA K = 64 int8)
+
+struct MfmaParams {
+  const uint8_t *E; // [T][F][D][2]
+  uint8_t *J;       // [B][F][2][T] or [B][F][T][2]
+  const v4i *W;     // [F][KS][NTP][64] weight operands
+  const int *corr;  // [F][BP][2]
+  const float *G;   // [F][BP]
+  int64_t T;
+  int32_t F, D, B;
+  int32_t NTP;       // 16-beam tiles over all beam groups (nbg * NT)
+  int32_t nbg;       // beam groups (one workgroup column each)
+  int32_t nchunks;   // time chunks per (f, beam group)
+  int32_t tiles_per_chunk;
+  int32_t nwork;     // F * nbg * nchunks
+  int32_t per_xcd;   // ceil(nwork / 8)
+  int32_t jlayout;   // 0 = [b][f][p][t], 1 = [b][f][t][p]
+};
+
+// Work index of a workgroup. Workgroups are dispatched round-robin over the
+// 8 XCDs; this gives each XCD a contiguous range of work items, so the beam
+// groups of one time chunk (which read the same E rows) and consecutive
+// chunks of one frequency (same weights) share that XCD's L2.
+BF_HD int work_of_block(int block, int per_xcd) { return (block & 7) * per_xcd + (block >> 3); }
+
+struct WorkItem {
+  int f, bg, chunk;
+};
+BF_HD WorkItem decode_work(int work, int nbg, int nchunks) {
+  WorkItem w;
+  w.bg = work % nbg;
+  const int rest = work / nbg;
+  w.chunk = rest % nchunks;
+  w.f = rest / nchunks;
+  return w;
+}
+
+} // namespace bf

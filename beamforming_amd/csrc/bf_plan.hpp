@@ -1,0 +1,49 @@
+// bf_plan.hpp -- the plan object behind the C ABI.
+#pragma once
+
+#include "beamform.h"
+#include "bf_kernels.hpp"
+
+#include <string>
+
+namespace bf {
+
+// MFMA kernel configuration for one shape.
+struct MfmaConfig {
+  int ks = 0;              // K steps (power of two, 32 dishes each)
+  int nt = 0;              // 16-beam tiles per workgroup
+  int nbg = 0;             // beam groups
+  int ntp = 0;             // nt * nbg
+  int tiles_per_chunk = 0; // 16-sample tiles per workgroup
+  int nchunks = 0;         // time chunks per (f, beam group)
+  int nwork = 0;           // workgroups with work
+  int per_xcd = 0;         // work items per XCD
+  int grid = 0;            // launched workgroups (8 * per_xcd)
+  size_t lds_bytes = 0;    // dynamic LDS per workgroup
+  MfmaKernel kernel = nullptr;
+};
+
+} // namespace bf
+
+struct bf_plan {
+  bf_shape shape{};
+  int device = 0;
+  int num_cus = 0;
+  bool fast_gains = false; // all gains inside the fast requantisation range
+  bool use_mfma = false;   // MFMA kernel (else the generic kernel)
+  bf::MfmaConfig cfg;
+  uint8_t *A_dev = nullptr; // [F][B][D] as given
+  float *G_dev = nullptr;   // [F][B] as given
+  bf::v4i *W = nullptr;     // [F][KS][NTP][64] weight operands
+  int *corr = nullptr;      // [F][BP][2]
+  float *Gp = nullptr;      // [F][BP]
+};
+
+namespace bf {
+extern thread_local std::string g_last_error;
+int fail(int status, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+int check_shape(const bf_shape *s);
+bool choose_mfma_config(const bf_shape &s, int num_cus, MfmaConfig *cfg);
+int launch_form_beams(const bf_plan *p, uint8_t *J, const uint8_t *E, hipStream_t st);
+} // namespace bf

@@ -1,0 +1,135 @@
+/*
+ * beamform.h -- C ABI of the MI355X (gfx950) form-beams library
+ * (libbeamform_amd.so).
+ *
+ * The reference computes, for every frequency f, beam b, polarization p and
+ * time t,
+ *     J[b,f,p,t] = requant( G[f,b] * sum_d A[f,b,d] * E[t,f,d,p] )
+ * on 4-bit complex values (reference cpu.cxx:15-46). Its entry point is
+ *     void form_beams(ucomplex4 *J, const ucomplex4 *E, const ucomplex4 *A,
+ *                     const float *G)                       (cpu.cxx:15-16)
+ * with shapes fixed at compile time (arraysizes.hxx:22-27) and, on the GPU,
+ * the same four pointers as device pointers (cuda.cu:35-36, launched at
+ * cuda.cu:291-296). This header replaces both: shapes are runtime values, the
+ * weights live in a plan, and every function returns a status instead of
+ * calling exit() (cuda.cu:255-262, arraysizes.cxx:82-86).
+ *
+ * Data formats (identical to the reference, byte for byte):
+ *   one complex value = one byte, ucomplex4 offset binary: high nibble = real,
+ *   low nibble = imaginary, each stored as value + 8 (icomplex4.hxx:22-40);
+ *   E[t][f][d][p]  (Elinear, arraysizes.hxx:79-89)
+ *   A[f][b][d]     (Alinear, arraysizes.hxx:114-123)
+ *   G[f][b] float  (Glinear, arraysizes.hxx:135-142)
+ *   J[b][f][p][t]  (Jlinear, arraysizes.hxx:91-101; BF_J_BFPT, the layout of
+ *                   every reference checksum) or
+ *   J[b][f][t][p]  (J2linear, arraysizes.hxx:103-112; BF_J_BFTP)
+ *
+ * Streams are hipStream_t values passed as void*; NULL is the null stream.
+ * All device pointers are plain hipMalloc'd (or torch) device memory.
+ */
+#ifndef BEAMFORM_H
+#define BEAMFORM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+enum {
+  BF_OK = 0,
+  BF_ERR_INVALID_ARGUMENT = 1, /* null pointer, negative size, bad layout */
+  BF_ERR_UNSUPPORTED = 2,      /* shape outside what the library handles */
+  BF_ERR_DEVICE = 3,           /* a HIP runtime call failed */
+  BF_ERR_OUT_OF_MEMORY = 4,    /* device or host allocation failed */
+  BF_ERR_OVER_BUDGET = 5,      /* serving: request larger than the HBM budget */
+  BF_ERR_NOT_READY = 6,        /* serving: no finished request to return */
+  BF_ERR_CLOSED = 7,           /* serving: server shut down */
+  BF_ERR_NO_DEVICE = 8         /* no HIP device visible */
+};
+
+/* J layouts. */
+enum { BF_J_BFPT = 0, BF_J_BFTP = 1 };
+
+typedef struct bf_shape {
+  int64_t ntimes;         /* T: time samples per chunk (arraysizes.hxx:22) */
+  int64_t nfrequencies;   /* F: frequency channels (arraysizes.hxx:23) */
+  int64_t ndishes;        /* D: dishes (arraysizes.hxx:24) */
+  int64_t npolarizations; /* P: must be 2 (arraysizes.hxx:25) */
+  int64_t nbeams;         /* B: beams (arraysizes.hxx:26) */
+  int32_t j_layout;       /* BF_J_BFPT (default) or BF_J_BFTP */
+  int32_t reserved;       /* must be 0 */
+} bf_shape;
+
+typedef struct bf_plan bf_plan;
+
+/* Human-readable text for a status code. */
+const char *bf_status_string(int status);
+
+/* Library version string. */
+const char *bf_version(void);
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char *bf_last_error(void);
+
+/* Creates a plan for `shape` with weights A (F*B*D bytes) and gains G (F*B
+ * floats) in HOST memory. The weights are expanded once on the device into
+ * the operand layout the kernels read and stay resident with the plan; the
+ * kernel configuration for the shape is chosen here. Replaces the per-call
+ * cudaMemcpy of A and G in cuda.cu:274-284. */
+int bf_plan_create(const bf_shape *shape, const uint8_t *A, const float *G, bf_plan **plan);
+
+/* Same, with A and G already in device memory (stream-ordered on `stream`;
+ * the call returns after the preparation has finished). */
+int bf_plan_create_device(const bf_shape *shape, const uint8_t *A_dev, const float *G_dev, void *stream,
+                          bf_plan **plan);
+
+/* Replaces a plan's weights and gains (host pointers), keeping its shape.
+ * Work already enqueued with the plan must have finished. */
+int bf_plan_update_weights(bf_plan *plan, const uint8_t *A, const float *G);
+
+/* Frees a plan. NULL is accepted. */
+int bf_plan_destroy(bf_plan *plan);
+
+/* Copies the plan's shape out. */
+int bf_plan_shape(const bf_plan *plan, bf_shape *shape);
+
+/* Writes a one-line description of the kernel configuration chosen for the
+ * plan (kernel family, tile sizes, grid) into buf. */
+int bf_plan_describe(const bf_plan *plan, char *buf, size_t len);
+
+/* form_beams on device buffers, asynchronous and ordered on `stream`:
+ * J (B*F*P*T bytes) <- E (T*F*D*P bytes). The GPU form of cpu.cxx:15-46 /
+ * cuda.cu:35-253; J is fully overwritten. A plan is immutable while in use,
+ * so calls on different streams may run concurrently (distinct J buffers). */
+int bf_form_beams_device(const bf_plan *plan, uint8_t *J, const uint8_t *E, void *stream);
+
+/* form_beams on host buffers: copies E in, runs, copies J out, synchronously
+ * (the flow of cuda.cu:274-314). */
+int bf_form_beams_host(const bf_plan *plan, uint8_t *J, const uint8_t *E);
+
+/* One-shot drop-in for the reference entry point with an explicit shape, all
+ * pointers in host memory: plan, run, destroy (cpu.cxx:15-16). */
+int bf_form_beams(uint8_t *J, const uint8_t *E, const uint8_t *A, const float *G, const bf_shape *shape);
+
+/* Adler-32 (adler32.c:7-20) of `len` bytes of DEVICE memory, computed on the
+ * device; the result is written to *out (host) after the stream has run it. */
+int bf_adler32_device(const uint8_t *data, size_t len, uint32_t *out, void *stream);
+
+/* Adler-32 of host memory (adler32.c:7-20), for the checksum flow of
+ * arraysizes.cxx:62-88 without a device round trip. */
+uint32_t bf_adler32(const uint8_t *data, size_t len);
+
+/* Fills `n` bytes of device memory with seeded uniform ucomplex4 values:
+ * byte i = enc(re, im), re = ((lo32(z) * 15) >> 32) - 7,
+ * im = ((hi32(z) * 15) >> 32) - 7, z = SplitMix64 output number (offset + i)
+ * for `seed`. Used to build large synthetic inputs in place. */
+int bf_fill_uniform_device(uint8_t *buf, size_t n, uint64_t seed, uint64_t offset, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEAMFORM_H */
