@@ -57,14 +57,14 @@ static bool gains_fast_safe(const float *G, int64_t n, int64_t D) {
   return true;
 }
 
-static MfmaKernel lookup_mfma(int nt, int ks) {
+static MfmaKernel lookup_mfma(int nt, int ks, bool dt) {
   switch (ks) {
-  case 1: return mfma_kernel_ks1(nt);
-  case 2: return mfma_kernel_ks2(nt);
-  case 4: return mfma_kernel_ks4(nt);
-  case 8: return mfma_kernel_ks8(nt);
-  case 16: return mfma_kernel_ks16(nt);
-  case 32: return mfma_kernel_ks32(nt);
+  case 2: return mfma_kernel_ks2(nt, dt);
+  case 4: return mfma_kernel_ks4(nt, dt);
+  case 8: return mfma_kernel_ks8(nt, dt);
+  case 16: return mfma_kernel_ks16(nt, dt);
+  case 32: return mfma_kernel_ks32(nt, dt);
+  case 64: return mfma_kernel_ks64(nt, dt);
   default: return nullptr;
   }
 }
@@ -74,11 +74,11 @@ static MfmaKernel lookup_mfma(int nt, int ks) {
 bool choose_mfma_config(const bf_shape &s, int num_cus, MfmaConfig *cfg) {
   if (s.npolarizations != 2 || s.ndishes % 8 != 0 || s.ndishes > 1024) return false;
   const int kneed = (int)((s.ndishes + kDishesPerStep - 1) / kDishesPerStep);
-  int ks = 1;
+  int ks = 2; // K steps are consumed in pairs
   while (ks < kneed) ks *= 2;
-  const int nt_tot = (int)((s.nbeams + 15) / 16);
-  static const int cand[] = {1, 2, 3, 4, 6};
-  const int max_nt = ks >= 32 ? 4 : 6; // LDS: ks * nt KiB; registers cap nt at 6
+  const int nt_tot = (int)((s.nbeams + kBeamTile - 1) / kBeamTile);
+  static const int cand[] = {1, 2, 3, 4};
+  const int max_nt = ks >= 64 ? 2 : 4; // LDS: ks * nt KiB <= 128 KiB; registers cap nt at 4
   // Fewest padded beam tiles, then fewest beam groups (E passes).
   int best_nt = 0, best_nbg = 0;
   for (int nt : cand) {
@@ -107,8 +107,9 @@ bool choose_mfma_config(const bf_shape &s, int num_cus, MfmaConfig *cfg) {
   cfg->nwork = (int)(cols * nchunks);
   cfg->per_xcd = (cfg->nwork + 7) / 8;
   cfg->grid = 8 * cfg->per_xcd;
-  cfg->lds_bytes = (size_t)ks * best_nt * 64 * sizeof(v4i);
-  cfg->kernel = lookup_mfma(best_nt, ks);
+  cfg->lds_bytes = (size_t)ks * best_nt * 64 * sizeof(v4i) + (size_t)best_nt * 32 * (sizeof(int) * 2 + sizeof(float));
+  cfg->dish_tail = (int64_t)ks * kDishesPerStep != s.ndishes;
+  cfg->kernel = lookup_mfma(best_nt, ks, cfg->dish_tail);
   return cfg->kernel != nullptr;
 }
 
@@ -132,7 +133,7 @@ static int prepare_weights(bf_plan *p, hipStream_t st) {
     const int blocks = (int)std::min<int64_t>((nw + 255) / 256, 65536);
     hipLaunchKernelGGL(prep_weights_kernel, dim3(blocks), dim3(256), 0, st, p->A_dev, p->W, F, B, D, c.ks, c.ntp);
     BF_HIP(hipGetLastError());
-    const int BP = c.ntp * 16;
+    const int BP = c.ntp * kBeamTile;
     const int64_t nc = (int64_t)F * BP;
     const int blocks2 = (int)std::min<int64_t>((nc + 255) / 256, 65536);
     hipLaunchKernelGGL(prep_corr_kernel, dim3(blocks2), dim3(256), 0, st, p->A_dev, p->G_dev, p->corr, p->Gp, F, B, D,
@@ -196,7 +197,7 @@ static int plan_create(const bf_shape *shape, const uint8_t *A, const float *G, 
   if (p->use_mfma) {
     const MfmaConfig &c = p->cfg;
     const size_t wbytes = (size_t)s.nfrequencies * c.ks * c.ntp * 64 * sizeof(v4i);
-    const size_t BP = (size_t)c.ntp * 16;
+    const size_t BP = (size_t)c.ntp * kBeamTile;
     if ((e = hipMalloc(&p->W, wbytes)) != hipSuccess) return bail(hip_fail(e, "hipMalloc W"));
     if ((e = hipMalloc(&p->corr, s.nfrequencies * BP * 2 * sizeof(int))) != hipSuccess)
       return bail(hip_fail(e, "hipMalloc corr"));
@@ -315,7 +316,7 @@ int bf_plan_describe(const bf_plan *p, char *buf, size_t len) {
   if (p->use_mfma) {
     const MfmaConfig &c = p->cfg;
     std::snprintf(buf, len,
-                  "mfma_i32_16x16x64_i8 nt=%d ks=%d nbg=%d tile_t=%d waves=%d tiles_per_chunk=%d nchunks=%d grid=%d "
+                  "mfma_i32_32x32x32_i8 nt=%d ks=%d nbg=%d tile_t=%d waves=%d tiles_per_chunk=%d nchunks=%d grid=%d "
                   "lds=%zu",
                   c.nt, c.ks, c.nbg, kTileT, kWaves, c.tiles_per_chunk, c.nchunks, c.grid, c.lds_bytes);
   } else {

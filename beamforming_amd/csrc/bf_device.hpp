@@ -36,10 +36,12 @@ BF_HD uint32_t enc(int re, int im) { return ((((uint32_t)re & 0xfu) << 4) | ((ui
 // The constant offsets depend only on (f, b); the accumulators start at their
 // negation, so they end at exactly raw_re / raw_im (exact int32 arithmetic).
 //
-// MFMA v_mfma_i32_16x16x64_i8 operand: each lane holds 16 int8 of one row
-// (E) or column (W). Per lane and K step the 16 k-values cover 8 dishes as
-// four dwords: [re d0-3 | im d0-3 | re d4-7 | im d4-7] on the weight side
-// and the matching v1 / v2 dwords on the E side.
+// MFMA v_mfma_i32_32x32x32_i8 operands: each lane holds 16 int8 of one row
+// (E: time and polarization) or column (W: beam). Per lane and K step the
+// 16 k-values cover 8 dishes as four dwords: [re d0-3 | re d4-7 | im d0-3 |
+// im d4-7] on the weight side. On the E side the nibble dwords
+// u = [lo_a lo_b hi_a hi_b nlo_a nlo_b] (a = dishes 0-3, b = 4-7) give both
+// variants as overlapping register windows: v2 = u[0..3], v1 = u[2..5].
 
 // Byte permute: result byte i = byte sel_i of the 8-byte value (hi:lo),
 // selectors 0-3 address lo, 4-7 address hi (v_perm_b32).
@@ -57,25 +59,24 @@ BF_HD uint32_t perm_bytes(uint32_t hi, uint32_t lo, uint32_t sel) {
 #endif
 }
 
-// Splits 16 packed E bytes (8 dishes x 2 polarizations: byte 2k+p = dish k,
-// polarization p) into the MFMA operands of both polarizations and both
-// variants: op[2p+0] = v1 of polarization p, op[2p+1] = v2.
-BF_HD void unpack_e(const v4i e, v4i op[4]) {
-  const uint32_t x0 = (uint32_t)e.x, x1 = (uint32_t)e.y, x2 = (uint32_t)e.z, x3 = (uint32_t)e.w;
-#pragma unroll
-  for (int p = 0; p < 2; ++p) {
-    const uint32_t sel = p == 0 ? 0x06040200u : 0x07050301u;
-    const uint32_t qa = perm_bytes(x1, x0, sel); // dishes 0-3 of polarization p
-    const uint32_t qb = perm_bytes(x3, x2, sel); // dishes 4-7
-    const uint32_t lo_a = qa & 0x0f0f0f0fu, hi_a = (qa >> 4) & 0x0f0f0f0fu;
-    const uint32_t lo_b = qb & 0x0f0f0f0fu, hi_b = (qb >> 4) & 0x0f0f0f0fu;
-    op[2 * p + 0] = v4i{(int)hi_a, (int)(lo_a ^ 0x0f0f0f0fu), (int)hi_b, (int)(lo_b ^ 0x0f0f0f0fu)};
-    op[2 * p + 1] = v4i{(int)lo_a, (int)hi_a, (int)lo_b, (int)hi_b};
-  }
+typedef int v8i __attribute__((ext_vector_type(8)));
+
+// Keeps one polarization of 16 packed E bytes (8 dishes x 2 polarizations:
+// byte 2k+p = dish k, polarization p; sel = 0x06040200 keeps p = 0,
+// 0x07050301 keeps p = 1) and splits its nibbles into the operand window
+// u = [lo_a lo_b hi_a hi_b nlo_a nlo_b . .]: v2 = u[0..3], v1 = u[2..5].
+BF_HD v8i unpack_e_pol(const v4i e, uint32_t sel) {
+  const uint32_t qa = perm_bytes((uint32_t)e.y, (uint32_t)e.x, sel); // dishes 0-3
+  const uint32_t qb = perm_bytes((uint32_t)e.w, (uint32_t)e.z, sel); // dishes 4-7
+  const uint32_t lo_a = qa & 0x0f0f0f0fu, hi_a = (qa >> 4) & 0x0f0f0f0fu;
+  const uint32_t lo_b = qb & 0x0f0f0f0fu, hi_b = (qb >> 4) & 0x0f0f0f0fu;
+  return v8i{(int)lo_a, (int)lo_b, (int)hi_a, (int)hi_b, (int)(lo_a ^ 0x0f0f0f0fu), (int)(lo_b ^ 0x0f0f0f0fu), 0, 0};
 }
+BF_HD v4i op_v1(const v8i u) { return __builtin_shufflevector(u, u, 2, 3, 4, 5); }
+BF_HD v4i op_v2(const v8i u) { return __builtin_shufflevector(u, u, 0, 1, 2, 3); }
 
 // Weight operand for one lane: 8 packed A bytes (dishes d0..d0+7, zero-padded
-// by the caller) -> [Ar d0-3 | Ai d0-3 | Ar d4-7 | Ai d4-7] as int8.
+// by the caller) -> [Ar d0-3 | Ar d4-7 | Ai d0-3 | Ai d4-7] as int8.
 BF_HD v4i pack_weights(const uint8_t a[8]) {
   uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
@@ -83,8 +84,8 @@ BF_HD v4i pack_weights(const uint8_t a[8]) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const uint32_t x = a[4 * q + i];
-      w[2 * q + 0] |= ((uint32_t)dec_re(x) & 0xffu) << (8 * i);
-      w[2 * q + 1] |= ((uint32_t)dec_im(x) & 0xffu) << (8 * i);
+      w[q + 0] |= ((uint32_t)dec_re(x) & 0xffu) << (8 * i);
+      w[q + 2] |= ((uint32_t)dec_im(x) & 0xffu) << (8 * i);
     }
   return v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
 }
@@ -143,18 +144,19 @@ BF_HD void corrections(int sum_ar, int sum_ai, int *corr_re, int *corr_im) {
 // Work decomposition of the MFMA kernel.
 constexpr int kWaves = 8;            // waves per workgroup (2 per SIMD)
 constexpr int kThreads = 64 * kWaves;
-constexpr int kTileT = 16;           // time samples per wave tile (MFMA M)
-constexpr int kDishesPerStep = 32;   // dishes per K step (MFMA K = 64 int8)
+constexpr int kTileT = 16;           // time samples per wave tile (x 2 polarizations = MFMA M)
+constexpr int kDishesPerStep = 16;   // dishes per K step (MFMA K = 32 int8)
+constexpr int kBeamTile = 32;        // beams per MFMA tile (MFMA N)
 
 struct MfmaParams {
   const uint8_t *E; // [T][F][D][2]
   uint8_t *J;       // [B][F][2][T] or [B][F][T][2]
   const v4i *W;     // [F][KS][NTP][64] weight operands
-  const int *corr;  // [F][BP][2]
+  const int *corr;  // [F][BP][2], BP = 32 NTP
   const float *G;   // [F][BP]
   int64_t T;
   int32_t F, D, B;
-  int32_t NTP;       // 16-beam tiles over all beam groups (nbg * NT)
+  int32_t NTP;       // 32-beam tiles over all beam groups (nbg * NT)
   int32_t nbg;       // beam groups (one workgroup column each)
   int32_t nchunks;   // time chunks per (f, beam group)
   int32_t tiles_per_chunk;

@@ -5,8 +5,8 @@
 namespace bf {
 
 // ---------------------------------------------------------------------------
-// Weight operands W[f][s][ntp][lane] (16 bytes each): lane = (g, r) holds
-// beam b = 16 ntp + r, dishes 32 s + 8 g .. +7, zero outside B and D.
+// Weight operands W[f][s][ntp][lane] (16 bytes each): lane = (g, c) holds
+// beam b = 32 ntp + c, dishes 16 s + 8 g .. +7, zero outside B and D.
 __global__ void prep_weights_kernel(const uint8_t *__restrict__ A, v4i *__restrict__ W, int F, int B, int D, int KS,
                                     int NTP) {
   const int64_t total = (int64_t)F * KS * NTP * 64;
@@ -17,8 +17,8 @@ __global__ void prep_weights_kernel(const uint8_t *__restrict__ A, v4i *__restri
     q /= NTP;
     const int s = (int)(q % KS);
     const int f = (int)(q / KS);
-    const int b = ntp * 16 + (lane & 15);
-    const int d0 = kDishesPerStep * s + 8 * (lane >> 4);
+    const int b = ntp * kBeamTile + (lane & 31);
+    const int d0 = kDishesPerStep * s + 8 * (lane >> 5);
     uint8_t a[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {

@@ -1,41 +1,91 @@
 // bf_mfma_kernel.hpp -- the gfx950 form-beams kernel (int8 MFMA, exact).
 //
-// Computes J for one (frequency, beam group, time chunk) per workgroup:
-//  * the prepared weight operands of the frequency's beam group (KS x NT
-//    KiB) are staged once into LDS and shared by the 8 waves;
-//  * each wave owns 16-sample time tiles; it streams the tile's packed E rows
-//    (1 byte per dish and polarization, 16 coalesced bytes per lane per K
-//    step) from HBM into a register ring that runs R K-steps ahead, splits
-//    them into nibble operands in registers, and runs
-//    v_mfma_i32_16x16x64_i8 against every beam tile in LDS: 4 MFMAs per
-//    (K step, beam tile) = 2 polarizations x (re, im) variants;
+// GEMM per frequency: rows = (16 time samples x 2 polarizations) per tile,
+// columns = beams (32 per MFMA tile), K = dishes x (re, im). The MFMA is
+// v_mfma_i32_32x32x32_i8, the full-rate int8 matrix instruction on gfx950.
+//
+// One workgroup = one (frequency, beam group, time chunk):
+//  * the prepared weight operands of the beam group (KS x NT KiB) are staged
+//    once into LDS and shared by the 8 waves (2 per SIMD);
+//  * each wave owns 16-sample time tiles. Lane (h, m) of the E operand is
+//    row m = (polarization m / 16, time m % 16) and k half h; per K step it
+//    loads the 16 packed E bytes of its time and dishes 16 s + 8 h .. +7
+//    (both polarizations; the lane pair m, m + 16 loads the same bytes) into
+//    a register ring running R steps ahead, keeps its polarization with one
+//    byte permute per 4 dishes and splits the nibbles into the two operand
+//    variants (see bf_device.hpp);
+//  * per K step and 32-beam tile: one LDS read of the weight operand and two
+//    MFMAs (real and imaginary accumulator);
 //  * the int32 accumulators start at minus the per-beam nibble offsets, so
 //    they end at the exact complex sums of cpu.cxx:24-35; the fused epilogue
 //    applies the float gain, round-to-nearest-even and the +-7 clamp
-//    (cpu.cxx:37-39), packs ucomplex4 bytes and stores J.
-// Rows past T and dishes past D are masked loads (zeros); the weights are
-// zero there, so they contribute nothing. The plan routes gains outside the
-// fast requantisation range (non-finite, or |G * raw| >= 2^31 possible) to
-// the generic kernel, which implements the exact x86 lrint semantics.
+//    (cpu.cxx:37-39), packs ucomplex4 bytes, swaps halves between lanes l and
+//    l ^ 32 so that each lane holds 16 consecutive times of one
+//    (beam, polarization) row, and stores 16 bytes per lane.
+// Rows past T and dishes past D are clamped, always-in-bounds loads whose
+// values are replaced by zeros; the weights there are zero too. The plan
+// routes gains outside the fast requantisation range (non-finite, or
+// |G * raw| >= 2^30 possible) to the generic kernel.
 #pragma once
 
 #include "bf_device.hpp"
 
+#include <type_traits>
+
 namespace bf {
 
-template <int NT, int KS>
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// Stores 16 bytes (w0..w3, little-endian) at dst with the widest accesses the
+// address alignment allows, keeping only bytes i < n.
+__device__ __forceinline__ void store_row16(uint8_t *dst, uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, int n) {
+  const uintptr_t al = (uintptr_t)dst;
+  if (n >= 16 && (al & 15) == 0) {
+    *reinterpret_cast<uint4 *>(dst) = make_uint4(w0, w1, w2, w3);
+  } else if (n >= 16 && (al & 3) == 0) {
+    uint32_t *d = reinterpret_cast<uint32_t *>(dst);
+    d[0] = w0;
+    d[1] = w1;
+    d[2] = w2;
+    d[3] = w3;
+  } else {
+    const uint32_t w[4] = {w0, w1, w2, w3};
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      if (i < n) dst[i] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+  }
+}
+
+// Stores 8 bytes (w0, w1) at dst, keeping bytes i < n.
+__device__ __forceinline__ void store_row8(uint8_t *dst, uint32_t w0, uint32_t w1, int n) {
+  if (n >= 8 && ((uintptr_t)dst & 7) == 0) {
+    *reinterpret_cast<uint2 *>(dst) = make_uint2(w0, w1);
+  } else {
+    const uint32_t w[2] = {w0, w1};
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (i < n) dst[i] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+  }
+}
+
+template <int NT, int KS, bool DT>
 __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) {
-  static_assert(NT >= 1 && NT <= 8, "beam tiles per workgroup");
-  static_assert(KS >= 1 && KS <= 32, "K steps");
-  constexpr int R = KS < 8 ? KS : 8; // E prefetch distance in K steps
-  extern __shared__ v4i Wl[];         // [KS][NT][64]
+  static_assert(NT >= 1 && NT <= 4, "32-beam tiles per workgroup");
+  static_assert(KS >= 2 && KS <= 64 && KS % 2 == 0, "K steps (pairs)");
+  constexpr int KP = KS / 2;                  // K-step pairs (one E load each)
+  constexpr int RP0 = NT >= 4 ? 4 : 8;        // E prefetch distance in pairs
+  constexpr int RP = RP0 < KP ? RP0 : KP;
+  static_assert(KP % RP == 0, "ring slots must line up across tiles");
+  extern __shared__ v4i Wl[]; // [KS][NT][64] weights, then per-beam data
+  // Per beam column of the group: nibble offsets and gains, after the weights.
+  int2 *Cl = reinterpret_cast<int2 *>(Wl + KS * NT * 64); // [NT * 32]
+  float *Gl = reinterpret_cast<float *>(Cl + NT * 32);     // [NT * 32]
 
   const int work = work_of_block(blockIdx.x, a.per_xcd);
   if (work >= a.nwork) return; // uniform per workgroup, before any barrier
   const WorkItem wi = decode_work(work, a.nbg, a.nchunks);
   const int f = wi.f;
 
-  // Stage the weight operands of (f, beam group) into LDS.
   {
     const v4i *src = a.W + ((size_t)f * KS * a.NTP + (size_t)wi.bg * NT) * 64;
     for (int i = threadIdx.x; i < KS * NT * 64; i += kThreads) {
@@ -43,13 +93,21 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
       const int rem = i - s * (NT * 64);
       Wl[i] = src[(size_t)s * a.NTP * 64 + rem];
     }
+    const size_t g0 = (size_t)f * a.NTP * 32 + (size_t)wi.bg * NT * 32;
+    for (int i = threadIdx.x; i < NT * 32; i += kThreads) {
+      Cl[i] = make_int2(a.corr[(g0 + i) * 2 + 0], a.corr[(g0 + i) * 2 + 1]);
+      Gl[i] = a.G[g0 + i];
+    }
   }
   __syncthreads();
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int g = lane >> 4;  // k group of this lane in the operands
-  const int r = lane & 15;  // E row (time) in the A operand; beam column in D
+  const int h = lane >> 5;          // k half (operands); row half (accumulators)
+  const int col = lane & 31;        // beam column of the accumulators
+  const int pm = (lane >> 4) & 1;   // polarization of this lane's E row
+  const int tm = lane & 15;         // time of this lane's E row within the tile
+  const uint32_t sel = pm ? 0x07050301u : 0x06040200u;
 
   const int64_t ntiles = (a.T + kTileT - 1) / kTileT;
   const int64_t tile_begin = (int64_t)wi.chunk * a.tiles_per_chunk;
@@ -57,131 +115,159 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
   int64_t tile = tile_begin + wave;
   if (tile >= tile_end) return; // no barrier follows
 
-  const int BP = a.NTP * 16;
-  float gain[NT];
-  int cre[NT], cim[NT];
-#pragma unroll
-  for (int nt = 0; nt < NT; ++nt) {
-    const int bp = (wi.bg * NT + nt) * 16 + r;
-    gain[nt] = a.G[(size_t)f * BP + bp];
-    cre[nt] = a.corr[((size_t)f * BP + bp) * 2 + 0];
-    cim[nt] = a.corr[((size_t)f * BP + bp) * 2 + 1];
-  }
-
-  // E row t of this frequency starts at E + (t F + f) D 2; this lane's 16
-  // bytes of K step s are dishes 32 s + 8 g .. +7, i.e. bytes 64 s + 16 g.
+  // E loads. Row t of this frequency starts at E + (t F + f) D 2. For K-step
+  // pair (2j, 2j + 1) lane (h, pm, tm) loads the 16 bytes of row tm at
+  // dishes 16 (2j + pm) + 8 h: byte 64 j + 32 pm + 16 h, so the four lanes
+  // of a row cover 64 contiguous bytes and no two lanes load the same bytes.
+  // v_permlane16_swap then gives every lane the bytes of both steps (lanes
+  // pm = 0 hold step 2j, lanes pm = 1 step 2j + 1; they are 16 lanes apart).
+  // Loads are unconditional and in bounds (rows clamped to T - 1; with DT,
+  // dishes past D read the row's first bytes); out-of-range values are zeroed where they
+  // are consumed, never right after the load.
   const int64_t e_tstride = (int64_t)a.F * a.D * 2;
-  const uint8_t *Ef = a.E + (int64_t)f * a.D * 2 + g * 16;
-  auto load_e = [&](int64_t tl, int s) -> v4i {
-    const int64_t t = tl * kTileT + r;
-    v4i v = {0, 0, 0, 0};
-    if (t < a.T && kDishesPerStep * s + 8 * g < a.D)
-      v = *reinterpret_cast<const v4i *>(Ef + t * e_tstride + 64 * s);
-    return v;
+  const uint8_t *Ef = a.E + (int64_t)f * a.D * 2 + pm * 32 + h * 16;
+  const int64_t tlast = a.T - 1;
+  auto row_ptr = [&](int64_t tl) -> const uint8_t * {
+    const int64_t t = tl * kTileT + tm;
+    return Ef + (t < a.T ? t : tlast) * e_tstride;
+  };
+  // dishes of K step s for the lanes of half h
+  auto dish_ok = [&](int s) -> bool { return kDishesPerStep * s + 8 * h < a.D; };
+  auto load_pair = [&](const uint8_t *p, int j) -> v4i {
+    if (DT) return *reinterpret_cast<const v4i *>(p + (dish_ok(2 * j + pm) ? 64 * j : -32 * pm - 16 * h));
+    return *reinterpret_cast<const v4i *>(p + 64 * j);
   };
 
-  v4i ring[R];
+  v4i ring[RP];
+  {
+    const uint8_t *p0 = row_ptr(tile);
 #pragma unroll
-  for (int q = 0; q < R; ++q) ring[q] = load_e(tile, q);
+    for (int q = 0; q < RP; ++q) ring[q] = load_pair(p0, q);
+  }
 
-  const bool aligned4 = (a.T & 3) == 0;
   for (; tile < tile_end; tile += kWaves) {
-    v4i acc[NT][4];
+    v16i acc[NT][2];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
-      acc[nt][0] = v4i{-cre[nt], -cre[nt], -cre[nt], -cre[nt]};
-      acc[nt][1] = v4i{-cim[nt], -cim[nt], -cim[nt], -cim[nt]};
-      acc[nt][2] = acc[nt][0];
-      acc[nt][3] = acc[nt][1];
-    }
-
-    // K loop: rounds of R steps; within a round the ring slot is static.
-#pragma unroll 1
-    for (int s0 = 0; s0 < KS; s0 += R) {
+      const int2 c = Cl[nt * 32 + col];
 #pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const int s = s0 + q;
-        const v4i e = ring[q];
-        {
-          const int s2 = s + R;
-          const int64_t tl2 = s2 >= KS ? tile + kWaves : tile;
-          const int ss2 = s2 >= KS ? s2 - KS : s2;
-          if (tl2 < tile_end) ring[q] = load_e(tl2, ss2);
-        }
-        v4i op[4];
-        unpack_e(e, op);
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-          const v4i w = Wl[(s * NT + nt) * 64 + lane];
-#pragma unroll
-          for (int v = 0; v < 4; ++v) acc[nt][v] = __builtin_amdgcn_mfma_i32_16x16x64_i8(op[v], w, acc[nt][v], 0, 0, 0);
-        }
+      for (int j = 0; j < 16; ++j) {
+        acc[nt][0][j] = -c.x;
+        acc[nt][1][j] = -c.y;
       }
     }
+    const bool row_ok = tile * kTileT + tm < a.T;
+    const uint8_t *pcur = row_ptr(tile);
+    const uint8_t *pnext = row_ptr(tile + kWaves);
 
-    // Epilogue. Lane (g, r) holds output rows 4g..4g+3 (times) of column r
-    // (beam) for every (beam tile, polarization, variant).
-    const int64_t t0 = tile * kTileT + 4 * g;
+    // One K loop in straight-line code (a scheduling region per step): the
+    // ring of loads in flight never crosses a loop back-edge inside a tile,
+    // so the waits before each use stay exact. MASK selects the variant that
+    // zeroes rows past T and dishes past D.
+    auto kloop = [&](auto mask_tag) {
+      constexpr bool MASK = decltype(mask_tag)::value;
+      v4i wcur[NT];
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) wcur[nt] = Wl[nt * 64 + lane];
+#pragma unroll
+      for (int j = 0; j < KP; ++j) {
+        const int q = j % RP;
+        const v4i raw = ring[q];
+        {
+          const int j2 = j + RP;
+          ring[q] = j2 >= KP ? load_pair(pnext, j2 - KP) : load_pair(pcur, j2);
+        }
+        v4i step[2]; // [0]: bytes of step 2j, [1]: step 2j + 1 (all lanes)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const auto sw = __builtin_amdgcn_permlane16_swap((unsigned)raw[k], (unsigned)raw[k], false, false);
+          step[0][k] = (int)sw[0];
+          step[1][k] = (int)sw[1];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int s = 2 * j + u;
+          v4i wnext[NT];
+          const int sn = s + 1 < KS ? s + 1 : s;
+#pragma unroll
+          for (int nt = 0; nt < NT; ++nt) wnext[nt] = Wl[(sn * NT + nt) * 64 + lane];
+          v4i e = step[u];
+          if (MASK) {
+            const bool ok = row_ok && (!DT || dish_ok(s));
+            e = ok ? e : v4i{0, 0, 0, 0};
+          }
+          const v8i uo = unpack_e_pol(e, sel);
+          const v4i v1 = op_v1(uo), v2 = op_v2(uo);
+#pragma unroll
+          for (int nt = 0; nt < NT; ++nt) {
+            acc[nt][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(v1, wcur[nt], acc[nt][0], 0, 0, 0);
+            acc[nt][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(v2, wcur[nt], acc[nt][1], 0, 0, 0);
+          }
+#pragma unroll
+          for (int nt = 0; nt < NT; ++nt) wcur[nt] = wnext[nt];
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+    };
+    if (DT || (tile + 1) * kTileT > a.T) // wave-uniform
+      kloop(std::true_type{});
+    else
+      kloop(std::false_type{});
+
+    // Epilogue. Accumulator element j of lane (h, col) is output row
+    // 8 (j / 4) + 4 h + j % 4 = (polarization j / 8, time 8 ((j / 4) & 1) +
+    // 4 h + j % 4) of beam column col.
+    const int64_t t0 = tile * kTileT;
+    const int nvalid = a.T - t0 < kTileT ? (int)(a.T - t0) : kTileT;
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
-      const int b = (wi.bg * NT + nt) * 16 + r;
-      if (b >= a.B) continue;
-      float byte[2][4];
+      const float gain = Gl[nt * 32 + col];
+      uint32_t wv[4]; // [p0: t 4h.., p0: t 8+4h.., p1: t 4h.., p1: t 8+4h..]
 #pragma unroll
-      for (int p = 0; p < 2; ++p)
+      for (int k = 0; k < 4; ++k) {
+        uint32_t w = 0;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const float qr = requant_fast(gain[nt], acc[nt][2 * p][j]);
-          const float qi = requant_fast(gain[nt], acc[nt][2 * p + 1][j]);
-          byte[p][j] = jbyte_f(qr, qi);
+        for (int i = 0; i < 4; ++i) {
+          const int j = 4 * k + i;
+          const float qr = requant_fast(gain, acc[nt][0][j]);
+          const float qi = requant_fast(gain, acc[nt][1][j]);
+          w = put_byte(jbyte_f(qr, qi), i, w);
         }
-      const bool full = aligned4 && t0 + 3 < a.T;
+        wv[k] = w;
+      }
+      const int b = (wi.bg * NT + nt) * 32 + col;
       if (a.jlayout == 0) {
-        // J[b][f][p][t]
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-          uint8_t *dst = a.J + (((int64_t)b * a.F + f) * 2 + p) * a.T + t0;
-          if (full) {
-            uint32_t w = 0;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) w = put_byte(byte[p][j], j, w);
-            *reinterpret_cast<uint32_t *>(dst) = w;
-          } else {
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-              if (t0 + j < a.T) dst[j] = (uint8_t)(int)byte[p][j];
-          }
+        // J[b][f][p][t]: lane h = 0 assembles polarization 0, lane h = 1
+        // polarization 1, exchanging the other halves with lane ^ 32.
+        const uint32_t x = h ? wv[0] : wv[2];
+        const uint32_t y = h ? wv[1] : wv[3];
+        const uint32_t xr = (uint32_t)__shfl_xor((int)x, 32, 64);
+        const uint32_t yr = (uint32_t)__shfl_xor((int)y, 32, 64);
+        if (b < a.B) {
+          uint8_t *dst = a.J + (((int64_t)b * a.F + f) * 2 + h) * a.T + t0;
+          if (h == 0)
+            store_row16(dst, wv[0], xr, wv[1], yr, nvalid);
+          else
+            store_row16(dst, xr, wv[2], yr, wv[3], nvalid);
         }
-      } else {
-        // J[b][f][t][p]
-        uint8_t *dst = a.J + (((int64_t)b * a.F + f) * a.T + t0) * 2;
-        if (full) {
-          uint32_t w0 = 0, w1 = 0;
-          w0 = put_byte(byte[0][0], 0, w0);
-          w0 = put_byte(byte[1][0], 1, w0);
-          w0 = put_byte(byte[0][1], 2, w0);
-          w0 = put_byte(byte[1][1], 3, w0);
-          w1 = put_byte(byte[0][2], 0, w1);
-          w1 = put_byte(byte[1][2], 1, w1);
-          w1 = put_byte(byte[0][3], 2, w1);
-          w1 = put_byte(byte[1][3], 3, w1);
-          *reinterpret_cast<uint2 *>(dst) = make_uint2(w0, w1);
-        } else {
+      } else if (b < a.B) {
+        // J[b][f][t][p]: interleave the polarizations of 4 times per half.
+        uint8_t *row = a.J + (((int64_t)b * a.F + f) * a.T + t0) * 2;
 #pragma unroll
-          for (int j = 0; j < 4; ++j)
-            if (t0 + j < a.T) {
-              dst[2 * j + 0] = (uint8_t)(int)byte[0][j];
-              dst[2 * j + 1] = (uint8_t)(int)byte[1][j];
-            }
+        for (int k = 0; k < 2; ++k) {
+          const int tt = 8 * k + 4 * h; // first time of this group in the tile
+          const uint32_t lo = perm_bytes(wv[2 + k], wv[k], 0x05010400u);
+          const uint32_t hi = perm_bytes(wv[2 + k], wv[k], 0x07030602u);
+          store_row8(row + 2 * tt, lo, hi, 2 * (nvalid - tt));
         }
       }
     }
   }
 }
 
-// Kernel table: one instance per (NT, KS) pair, registered by the
-// bf_mfma_ks*.hip translation units.
+// Kernel table: one instance per (NT, KS, DT), registered by the
+// bf_mfma_ks*.hip translation units. DT = true handles ndishes that are not
+// KS * 16 (the last K steps are partly or wholly past D).
 typedef void (*MfmaKernel)(const MfmaParams);
-MfmaKernel mfma_kernel_for(int nt, int ks);
 
 } // namespace bf

@@ -1,22 +1,18 @@
-// bf_mfma_ks2.hip -- MFMA form-beams kernel instances with 2 K steps (64 dishes).
+// bf_mfma_ks2.hip -- MFMA form-beams kernel instances with 2 K steps (32 dishes).
 #include "bf_kernels.hpp"
 
 namespace bf {
 
-MfmaKernel mfma_kernel_ks2(int nt) {
+MfmaKernel mfma_kernel_ks2(int nt, bool dish_tail) {
   switch (nt) {
   case 1:
-    return form_beams_mfma<1, 2>;
+    return dish_tail ? form_beams_mfma<1, 2, true> : form_beams_mfma<1, 2, false>;
   case 2:
-    return form_beams_mfma<2, 2>;
+    return dish_tail ? form_beams_mfma<2, 2, true> : form_beams_mfma<2, 2, false>;
   case 3:
-    return form_beams_mfma<3, 2>;
+    return dish_tail ? form_beams_mfma<3, 2, true> : form_beams_mfma<3, 2, false>;
   case 4:
-    return form_beams_mfma<4, 2>;
-  case 6:
-    return form_beams_mfma<6, 2>;
-  case 8:
-    return form_beams_mfma<8, 2>;
+    return dish_tail ? form_beams_mfma<4, 2, true> : form_beams_mfma<4, 2, false>;
   default:
     return nullptr;
   }

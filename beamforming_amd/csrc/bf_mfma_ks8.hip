@@ -1,22 +1,18 @@
-// bf_mfma_ks8.hip -- MFMA form-beams kernel instances with 8 K steps (256 dishes).
+// bf_mfma_ks8.hip -- MFMA form-beams kernel instances with 8 K steps (128 dishes).
 #include "bf_kernels.hpp"
 
 namespace bf {
 
-MfmaKernel mfma_kernel_ks8(int nt) {
+MfmaKernel mfma_kernel_ks8(int nt, bool dish_tail) {
   switch (nt) {
   case 1:
-    return form_beams_mfma<1, 8>;
+    return dish_tail ? form_beams_mfma<1, 8, true> : form_beams_mfma<1, 8, false>;
   case 2:
-    return form_beams_mfma<2, 8>;
+    return dish_tail ? form_beams_mfma<2, 8, true> : form_beams_mfma<2, 8, false>;
   case 3:
-    return form_beams_mfma<3, 8>;
+    return dish_tail ? form_beams_mfma<3, 8, true> : form_beams_mfma<3, 8, false>;
   case 4:
-    return form_beams_mfma<4, 8>;
-  case 6:
-    return form_beams_mfma<6, 8>;
-  case 8:
-    return form_beams_mfma<8, 8>;
+    return dish_tail ? form_beams_mfma<4, 8, true> : form_beams_mfma<4, 8, false>;
   default:
     return nullptr;
   }

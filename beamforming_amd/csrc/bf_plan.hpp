@@ -10,7 +10,8 @@ namespace bf {
 
 // MFMA kernel configuration for one shape.
 struct MfmaConfig {
-  int ks = 0;              // K steps (power of two, 32 dishes each)
+  int ks = 0;              // K steps (power of two, 16 dishes each)
+  bool dish_tail = false;  // ks * 16 != ndishes: masked last steps
   int nt = 0;              // 16-beam tiles per workgroup
   int nbg = 0;             // beam groups
   int ntp = 0;             // nt * nbg
