@@ -107,7 +107,8 @@ bool choose_mfma_config(const bf_shape &s, int num_cus, MfmaConfig *cfg) {
   cfg->nwork = (int)(cols * nchunks);
   cfg->per_xcd = (cfg->nwork + 7) / 8;
   cfg->grid = 8 * cfg->per_xcd;
-  cfg->lds_bytes = (size_t)ks * best_nt * 64 * sizeof(v4i) + (size_t)best_nt * 32 * (sizeof(int) * 2 + sizeof(float));
+  cfg->lds_bytes = (size_t)ks * best_nt * 64 * sizeof(v4i) + (size_t)best_nt * 32 * (sizeof(int) * 2 + sizeof(float)) +
+                   (size_t)kWaves * 64 * sizeof(v4i);
   cfg->dish_tail = (int64_t)ks * kDishesPerStep != s.ndishes;
   cfg->kernel = lookup_mfma(best_nt, ks, cfg->dish_tail);
   return cfg->kernel != nullptr;

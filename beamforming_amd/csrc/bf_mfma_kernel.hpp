@@ -73,13 +73,15 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
   static_assert(NT >= 1 && NT <= 4, "32-beam tiles per workgroup");
   static_assert(KS >= 2 && KS <= 64 && KS % 2 == 0, "K steps (pairs)");
   constexpr int KP = KS / 2;                  // K-step pairs (one E load each)
-  constexpr int RP0 = NT >= 4 ? 4 : 8;        // E prefetch distance in pairs
+  constexpr int RP0 = NT >= 4 ? 8 : 16;       // E prefetch distance in pairs
   constexpr int RP = RP0 < KP ? RP0 : KP;
   static_assert(KP % RP == 0, "ring slots must line up across tiles");
   extern __shared__ v4i Wl[]; // [KS][NT][64] weights, then per-beam data
   // Per beam column of the group: nibble offsets and gains, after the weights.
   int2 *Cl = reinterpret_cast<int2 *>(Wl + KS * NT * 64); // [NT * 32]
   float *Gl = reinterpret_cast<float *>(Cl + NT * 32);     // [NT * 32]
+  // Per-wave E transpose buffers: 16 rows x 64 bytes each.
+  v4i *Xl = reinterpret_cast<v4i *>(Gl + NT * 32);        // [kWaves][16][4]
 
   const int work = work_of_block(blockIdx.x, a.per_xcd);
   if (work >= a.nwork) return; // uniform per workgroup, before any barrier
@@ -115,28 +117,36 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
   int64_t tile = tile_begin + wave;
   if (tile >= tile_end) return; // no barrier follows
 
-  // E loads. Row t of this frequency starts at E + (t F + f) D 2. For K-step
-  // pair (2j, 2j + 1) lane (h, pm, tm) loads the 16 bytes of row tm at
-  // dishes 16 (2j + pm) + 8 h: byte 64 j + 32 pm + 16 h, so the four lanes
-  // of a row cover 64 contiguous bytes and no two lanes load the same bytes.
-  // v_permlane16_swap then gives every lane the bytes of both steps (lanes
-  // pm = 0 hold step 2j, lanes pm = 1 step 2j + 1; they are 16 lanes apart).
+  // E loads, fully coalesced. Row t of this frequency starts at
+  // E + (t F + f) D 2. For K-step pair (2j, 2j + 1) a tile needs bytes
+  // 64 j .. 64 j + 63 of its 16 rows: lane l loads the 16-byte piece l & 3
+  // of row l >> 2, so every lane quad reads one contiguous 64-byte segment.
+  // The wave then transposes through its 1 KiB LDS buffer: the operand of
+  // step 2j + u for lane (h, pm, tm) is piece 2u + h of row tm (dishes
+  // 16 (2j + u) + 8 h .. +7). Pieces are stored at slot (piece + row / 2) & 3
+  // of their row so that the reads of 8 consecutive rows hit distinct banks.
   // Loads are unconditional and in bounds (rows clamped to T - 1; with DT,
-  // dishes past D read the row's first bytes); out-of-range values are zeroed where they
-  // are consumed, never right after the load.
+  // dishes past D read the row's first bytes); out-of-range values are zeroed
+  // where they are consumed, never right after the load.
+  const int ld_row = lane >> 2, ld_piece = lane & 3;
   const int64_t e_tstride = (int64_t)a.F * a.D * 2;
-  const uint8_t *Ef = a.E + (int64_t)f * a.D * 2 + pm * 32 + h * 16;
+  const uint8_t *Ef = a.E + (int64_t)f * a.D * 2 + ld_piece * 16;
   const int64_t tlast = a.T - 1;
   auto row_ptr = [&](int64_t tl) -> const uint8_t * {
-    const int64_t t = tl * kTileT + tm;
+    const int64_t t = tl * kTileT + ld_row;
     return Ef + (t < a.T ? t : tlast) * e_tstride;
   };
-  // dishes of K step s for the lanes of half h
+  // dishes of K step s for the operand lanes of half h
   auto dish_ok = [&](int s) -> bool { return kDishesPerStep * s + 8 * h < a.D; };
   auto load_pair = [&](const uint8_t *p, int j) -> v4i {
-    if (DT) return *reinterpret_cast<const v4i *>(p + (dish_ok(2 * j + pm) ? 64 * j : -32 * pm - 16 * h));
+    if (DT) return *reinterpret_cast<const v4i *>(p + (32 * j + 8 * ld_piece < a.D ? 64 * j : -16 * ld_piece));
     return *reinterpret_cast<const v4i *>(p + 64 * j);
   };
+  v4i *xw = Xl + wave * 64;
+  const int wr_slot = ld_row * 4 + ((ld_piece + (ld_row >> 1)) & 3);
+  int rd_slot[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) rd_slot[u] = tm * 4 + ((2 * u + h + (tm >> 1)) & 3);
 
   v4i ring[RP];
   {
@@ -156,57 +166,61 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
         acc[nt][1][j] = -c.y;
       }
     }
-    const bool row_ok = tile * kTileT + tm < a.T;
+    const bool row_ok = tile * kTileT + tm < a.T; // row of this lane's operand
     const uint8_t *pcur = row_ptr(tile);
     const uint8_t *pnext = row_ptr(tile + kWaves);
 
     // One K loop in straight-line code (a scheduling region per step): the
     // ring of loads in flight never crosses a loop back-edge inside a tile,
-    // so the waits before each use stay exact. MASK selects the variant that
-    // zeroes rows past T and dishes past D.
+    // so the waits before each use stay exact. The loop is software
+    // pipelined: during step s the wave unpacks the operands of step s + 1
+    // (independent VALU work that overlaps the step's MFMAs; grouped after
+    // them rather than interleaved, which measures faster). MASK selects the
+    // variant that zeroes rows past T and dishes past D.
     auto kloop = [&](auto mask_tag) {
       constexpr bool MASK = decltype(mask_tag)::value;
-      v4i wcur[NT];
-#pragma unroll
-      for (int nt = 0; nt < NT; ++nt) wcur[nt] = Wl[nt * 64 + lane];
-#pragma unroll
-      for (int j = 0; j < KP; ++j) {
-        const int q = j % RP;
-        const v4i raw = ring[q];
-        {
-          const int j2 = j + RP;
-          ring[q] = j2 >= KP ? load_pair(pnext, j2 - KP) : load_pair(pcur, j2);
+      auto operands = [&](v4i e, int s, v4i &v1, v4i &v2) {
+        if (MASK) {
+          const bool ok = row_ok && (!DT || dish_ok(s));
+          e = ok ? e : v4i{0, 0, 0, 0};
         }
-        v4i step[2]; // [0]: bytes of step 2j, [1]: step 2j + 1 (all lanes)
+        const v8i uo = unpack_e_pol(e, sel);
+        v1 = op_v1(uo);
+        v2 = op_v2(uo);
+      };
+      // prologue: pair 0 through the transpose buffer, operands of step 0
+      v4i v1c, v2c;
+      {
+        xw[wr_slot] = ring[0];
+        ring[0] = RP < KP ? load_pair(pcur, RP) : load_pair(pnext, RP - KP);
+        operands(xw[rd_slot[0]], 0, v1c, v2c);
+      }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const auto sw = __builtin_amdgcn_permlane16_swap((unsigned)raw[k], (unsigned)raw[k], false, false);
-          step[0][k] = (int)sw[0];
-          step[1][k] = (int)sw[1];
-        }
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int s = 2 * j + u;
-          v4i wnext[NT];
-          const int sn = s + 1 < KS ? s + 1 : s;
-#pragma unroll
-          for (int nt = 0; nt < NT; ++nt) wnext[nt] = Wl[(sn * NT + nt) * 64 + lane];
-          v4i e = step[u];
-          if (MASK) {
-            const bool ok = row_ok && (!DT || dish_ok(s));
-            e = ok ? e : v4i{0, 0, 0, 0};
+      for (int s = 0; s < KS; ++s) {
+        const bool more = s + 1 < KS;
+        v4i v1n = v1c, v2n = v2c;
+        if (more) {
+          if (s & 1) {
+            // next step starts pair j + 1: stage it, refill its ring slot
+            const int j1 = (s + 1) / 2;
+            const int q = j1 % RP;
+            xw[wr_slot] = ring[q]; // the wave's LDS operations complete in order
+            const int j2 = j1 + RP;
+            ring[q] = j2 >= KP ? load_pair(pnext, j2 - KP) : load_pair(pcur, j2);
           }
-          const v8i uo = unpack_e_pol(e, sel);
-          const v4i v1 = op_v1(uo), v2 = op_v2(uo);
-#pragma unroll
-          for (int nt = 0; nt < NT; ++nt) {
-            acc[nt][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(v1, wcur[nt], acc[nt][0], 0, 0, 0);
-            acc[nt][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(v2, wcur[nt], acc[nt][1], 0, 0, 0);
-          }
-#pragma unroll
-          for (int nt = 0; nt < NT; ++nt) wcur[nt] = wnext[nt];
-          __builtin_amdgcn_sched_barrier(0);
+          operands(xw[rd_slot[(s + 1) & 1]], s + 1, v1n, v2n);
         }
+        v4i w[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) w[nt] = Wl[(s * NT + nt) * 64 + lane];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          acc[nt][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(v1c, w[nt], acc[nt][0], 0, 0, 0);
+          acc[nt][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(v2c, w[nt], acc[nt][1], 0, 0, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        v1c = v1n;
+        v2c = v2n;
       }
     };
     if (DT || (tile + 1) * kTileT > a.T) // wave-uniform
@@ -219,10 +233,10 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
     // 4 h + j % 4) of beam column col.
     const int64_t t0 = tile * kTileT;
     const int nvalid = a.T - t0 < kTileT ? (int)(a.T - t0) : kTileT;
+    uint32_t wv[NT][4]; // [p0: t 4h.., p0: t 8+4h.., p1: t 4h.., p1: t 8+4h..]
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
       const float gain = Gl[nt * 32 + col];
-      uint32_t wv[4]; // [p0: t 4h.., p0: t 8+4h.., p1: t 4h.., p1: t 8+4h..]
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         uint32_t w = 0;
@@ -233,22 +247,29 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
           const float qi = requant_fast(gain, acc[nt][1][j]);
           w = put_byte(jbyte_f(qr, qi), i, w);
         }
-        wv[k] = w;
+        wv[nt][k] = w;
       }
+    }
+    // The E loads of the next tile's first steps were issued thousands of
+    // cycles ago; completing them here, before the stores, keeps the stores
+    // (counted by the same vmcnt) out of every wait of the next tile.
+    __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
       const int b = (wi.bg * NT + nt) * 32 + col;
       if (a.jlayout == 0) {
         // J[b][f][p][t]: lane h = 0 assembles polarization 0, lane h = 1
         // polarization 1, exchanging the other halves with lane ^ 32.
-        const uint32_t x = h ? wv[0] : wv[2];
-        const uint32_t y = h ? wv[1] : wv[3];
+        const uint32_t x = h ? wv[nt][0] : wv[nt][2];
+        const uint32_t y = h ? wv[nt][1] : wv[nt][3];
         const uint32_t xr = (uint32_t)__shfl_xor((int)x, 32, 64);
         const uint32_t yr = (uint32_t)__shfl_xor((int)y, 32, 64);
         if (b < a.B) {
           uint8_t *dst = a.J + (((int64_t)b * a.F + f) * 2 + h) * a.T + t0;
           if (h == 0)
-            store_row16(dst, wv[0], xr, wv[1], yr, nvalid);
+            store_row16(dst, wv[nt][0], xr, wv[nt][1], yr, nvalid);
           else
-            store_row16(dst, xr, wv[2], yr, wv[3], nvalid);
+            store_row16(dst, xr, wv[nt][2], yr, wv[nt][3], nvalid);
         }
       } else if (b < a.B) {
         // J[b][f][t][p]: interleave the polarizations of 4 times per half.
@@ -256,8 +277,8 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
           const int tt = 8 * k + 4 * h; // first time of this group in the tile
-          const uint32_t lo = perm_bytes(wv[2 + k], wv[k], 0x05010400u);
-          const uint32_t hi = perm_bytes(wv[2 + k], wv[k], 0x07030602u);
+          const uint32_t lo = perm_bytes(wv[nt][2 + k], wv[nt][k], 0x05010400u);
+          const uint32_t hi = perm_bytes(wv[nt][2 + k], wv[nt][k], 0x07030602u);
           store_row8(row + 2 * tt, lo, hi, 2 * (nvalid - tt));
         }
       }
