@@ -18,7 +18,7 @@ import numpy as np
 
 __all__ = [
     "BF_J_BFPT", "BF_J_BFTP", "BeamformError", "Shape", "Plan", "lib", "library_path",
-    "form_beams", "adler32", "adler32_device", "fill_uniform_device", "version",
+    "form_beams", "adler32", "adler32_device", "fill_uniform_device", "version", "Chunk", "Batch",
 ]
 
 BF_J_BFPT = 0  # J[b][f][p][t] (reference Jlinear, arraysizes.hxx:91-101)
@@ -73,6 +73,18 @@ class Shape(ctypes.Structure):
         return (self.ntimes, self.nfrequencies, self.ndishes, self.npolarizations, self.nbeams, self.j_layout)
 
 
+class Chunk(ctypes.Structure):
+    """bf_chunk: one chunk of a batch (device E and J, its ntimes and channel)."""
+
+    _fields_ = [
+        ("E", ctypes.c_void_p),
+        ("J", ctypes.c_void_p),
+        ("ntimes", ctypes.c_int64),
+        ("channel", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 def _load():
     if not os.path.exists(library_path):
         raise ImportError(f"{library_path} is not built; run `make -C {_HERE}` (hipcc, gfx950)")
@@ -94,6 +106,16 @@ def _load():
         "bf_adler32_device": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32), vp]),
         "bf_adler32": (ctypes.c_uint32, [u8p, ctypes.c_size_t]),
         "bf_fill_uniform_device": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64, vp]),
+        "bf_form_beams_channel": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int64, u8p, u8p, vp]),
+        "bf_batch_create": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(vp)]),
+        "bf_batch_destroy": (ctypes.c_int, [vp]),
+        "bf_batch_set_item_tiles": (ctypes.c_int, [vp, ctypes.c_int]),
+        "bf_batch_run": (ctypes.c_int, [vp, ctypes.POINTER(Chunk), ctypes.c_int, vp]),
+        "bf_batch_run_graph": (ctypes.c_int, [vp, ctypes.POINTER(Chunk), ctypes.c_int, vp]),
+        "bf_batch_describe": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_size_t]),
+        "bf_plan_autotune": (ctypes.c_int, [vp, u8p, u8p, vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]),
+        "bf_batch_autotune": (ctypes.c_int, [vp, ctypes.POINTER(Chunk), ctypes.c_int, vp, ctypes.c_int,
+                                             ctypes.c_char_p, ctypes.c_size_t]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -172,6 +194,18 @@ class Plan:
         _check(lib.bf_form_beams_device(self._h, ctypes.c_void_p(J_ptr), ctypes.c_void_p(E_ptr),
                                         ctypes.c_void_p(stream)))
 
+    def form_beams_channel(self, channel, ntimes, J_ptr, E_ptr, stream=0):
+        """One channel of a channel-table plan, with its own ntimes (device pointers)."""
+        _check(lib.bf_form_beams_channel(self._h, channel, ntimes, ctypes.c_void_p(J_ptr), ctypes.c_void_p(E_ptr),
+                                         ctypes.c_void_p(stream)))
+
+    def autotune(self, J_ptr, E_ptr, stream=0, iters=0):
+        """Times candidate configurations on the given device buffers, keeps the best; returns the report."""
+        buf = ctypes.create_string_buffer(4096)
+        _check(lib.bf_plan_autotune(self._h, ctypes.c_void_p(J_ptr), ctypes.c_void_p(E_ptr), ctypes.c_void_p(stream),
+                                    iters, buf, 4096))
+        return buf.value.decode()
+
     def form_beams_host(self, E):
         """Synchronous: host E (numpy uint8) -> host J (numpy uint8)."""
         E_arr, E_p = _host_ptr(E, np.uint8)
@@ -208,3 +242,61 @@ def adler32_device(ptr, nbytes, stream=0):
 def fill_uniform_device(ptr, nbytes, seed, offset=0, stream=0):
     """Seeded uniform ucomplex4 bytes into device memory (same stream as host oracle)."""
     _check(lib.bf_fill_uniform_device(ctypes.c_void_p(ptr), nbytes, seed, offset, ctypes.c_void_p(stream)))
+
+
+def _chunk_array(chunks):
+    arr = (Chunk * max(1, len(chunks)))()
+    for i, c in enumerate(chunks):
+        if isinstance(c, Chunk):
+            arr[i] = c
+        else:
+            E, J, ntimes, channel = c
+            arr[i] = Chunk(E, J, ntimes, channel, 0)
+    return arr
+
+
+class Batch:
+    """A batch context over a channel-table plan: many chunks (each one
+    channel, own ntimes) per persistent launch with a device-side queue."""
+
+    def __init__(self, plan, max_chunks):
+        self.plan = plan  # keeps the plan alive
+        self._h = ctypes.c_void_p()
+        _check(lib.bf_batch_create(plan.handle, max_chunks, ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib.bf_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_item_tiles(self, n):
+        _check(lib.bf_batch_set_item_tiles(self._h, n))
+
+    def run(self, chunks, stream=0, graph=False):
+        """chunks: list of Chunk or (E_ptr, J_ptr, ntimes, channel)."""
+        arr = _chunk_array(chunks)
+        fn = lib.bf_batch_run_graph if graph else lib.bf_batch_run
+        _check(fn(self._h, arr, len(chunks), ctypes.c_void_p(stream)))
+
+    def autotune(self, chunks, stream=0, iters=0):
+        arr = _chunk_array(chunks)
+        buf = ctypes.create_string_buffer(4096)
+        _check(lib.bf_batch_autotune(self._h, arr, len(chunks), ctypes.c_void_p(stream), iters, buf, 4096))
+        return buf.value.decode()
+
+    def describe(self):
+        buf = ctypes.create_string_buffer(512)
+        _check(lib.bf_batch_describe(self._h, buf, 512))
+        return buf.value.decode()

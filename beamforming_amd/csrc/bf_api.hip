@@ -12,6 +12,9 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <array>
+#include <map>
+#include <mutex>
 #include <vector>
 
 namespace bf {
@@ -69,49 +72,118 @@ static MfmaKernel lookup_mfma(int nt, int ks, bool dt) {
   }
 }
 
-// Chooses the MFMA configuration for a shape. Returns false if the shape is
-// outside the MFMA kernel's range.
-bool choose_mfma_config(const bf_shape &s, int num_cus, MfmaConfig *cfg) {
-  if (s.npolarizations != 2 || s.ndishes % 8 != 0 || s.ndishes > 1024) return false;
-  const int kneed = (int)((s.ndishes + kDishesPerStep - 1) / kDishesPerStep);
+BatchKernel lookup_batch(int nt, int ks, bool dt) {
+  switch (ks) {
+  case 2: return batch_kernel_ks2(nt, dt);
+  case 4: return batch_kernel_ks4(nt, dt);
+  case 8: return batch_kernel_ks8(nt, dt);
+  case 16: return batch_kernel_ks16(nt, dt);
+  case 32: return batch_kernel_ks32(nt, dt);
+  case 64: return batch_kernel_ks64(nt, dt);
+  default: return nullptr;
+  }
+}
+
+// Tuned choices per shape, filled by bf_plan_autotune and consulted by later
+// plans of the same shape in this process.
+static std::mutex g_tune_mutex;
+static std::map<std::array<int64_t, 5>, TuneChoice> g_tune_cache;
+
+static std::array<int64_t, 5> tune_key(const bf_shape &s) {
+  return {s.ntimes, s.nfrequencies, s.ndishes, s.nbeams, s.j_layout};
+}
+
+void remember_tuning(const bf_shape &s, const TuneChoice &c) {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  g_tune_cache[tune_key(s)] = c;
+}
+
+bool recall_tuning(const bf_shape &s, TuneChoice *c) {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  auto it = g_tune_cache.find(tune_key(s));
+  if (it == g_tune_cache.end()) return false;
+  *c = it->second;
+  return true;
+}
+
+// Beam tiling (KS, NT, beam groups) for D and B. forced_nt > 0 selects NT.
+bool choose_tiling(int64_t ndishes, int64_t nbeams, int forced_nt, MfmaConfig *cfg) {
+  const int kneed = (int)((ndishes + kDishesPerStep - 1) / kDishesPerStep);
   int ks = 2; // K steps are consumed in pairs
   while (ks < kneed) ks *= 2;
-  const int nt_tot = (int)((s.nbeams + kBeamTile - 1) / kBeamTile);
-  static const int cand[] = {1, 2, 3, 4};
+  if (ks > 64) return false;
+  const int nt_tot = (int)((nbeams + kBeamTile - 1) / kBeamTile);
   const int max_nt = ks >= 64 ? 2 : 4; // LDS: ks * nt KiB <= 128 KiB; registers cap nt at 4
-  // Fewest padded beam tiles, then fewest beam groups (E passes).
   int best_nt = 0, best_nbg = 0;
-  for (int nt : cand) {
-    if (nt > max_nt) continue;
-    const int nbg = (nt_tot + nt - 1) / nt;
-    const int padded = nbg * nt;
-    if (best_nt == 0 || padded < best_nt * best_nbg || (padded == best_nt * best_nbg && nbg < best_nbg)) {
-      best_nt = nt;
-      best_nbg = nbg;
+  if (forced_nt > 0) {
+    if (forced_nt > max_nt) return false;
+    best_nt = forced_nt;
+    best_nbg = (nt_tot + forced_nt - 1) / forced_nt;
+  } else {
+    // Fewest padded beam tiles, then fewest beam groups (E passes).
+    for (int nt = 1; nt <= max_nt; ++nt) {
+      const int nbg = (nt_tot + nt - 1) / nt;
+      const int padded = nbg * nt;
+      if (best_nt == 0 || padded < best_nt * best_nbg || (padded == best_nt * best_nbg && nbg < best_nbg)) {
+        best_nt = nt;
+        best_nbg = nbg;
+      }
     }
   }
   cfg->ks = ks;
   cfg->nt = best_nt;
   cfg->nbg = best_nbg;
   cfg->ntp = best_nt * best_nbg;
-  const int64_t ntiles = (s.ntimes + kTileT - 1) / kTileT;
-  const int64_t cols = s.nfrequencies * best_nbg;
-  const int64_t target = std::max<int64_t>(num_cus, 1);
+  cfg->lds_bytes = (size_t)ks * best_nt * 64 * sizeof(v4i) + (size_t)best_nt * 32 * (sizeof(int) * 2 + sizeof(float)) +
+                   (size_t)kWaves * 64 * sizeof(v4i);
+  cfg->dish_tail = (int64_t)ks * kDishesPerStep != ndishes;
+  cfg->kernel = lookup_mfma(best_nt, ks, cfg->dish_tail);
+  cfg->batch_kernel = lookup_batch(best_nt, ks, cfg->dish_tail);
+  return cfg->kernel != nullptr && cfg->batch_kernel != nullptr;
+}
+
+// Grid for T time samples and F frequencies: about target_wgs workgroups, each
+// a multiple of 8 tiles (one per wave).
+bool choose_grid(int64_t ntimes, int64_t nfrequencies, int64_t target_wgs, MfmaConfig *cfg) {
+  const int64_t ntiles = (ntimes + kTileT - 1) / kTileT;
+  const int64_t cols = nfrequencies * cfg->nbg;
+  const int64_t target = std::max<int64_t>(target_wgs, 1);
   const int64_t chunks0 = std::max<int64_t>(1, (target + cols - 1) / cols);
   int64_t tpc = (ntiles + chunks0 - 1) / chunks0;
   tpc = (tpc + kWaves - 1) / kWaves * kWaves;
   const int64_t nchunks = (ntiles + tpc - 1) / tpc;
-  if (cols * nchunks > (1ll << 30)) return false;
+  if (cols * nchunks > (1ll << 30) || tpc > (1ll << 30)) return false;
   cfg->tiles_per_chunk = (int)tpc;
   cfg->nchunks = (int)nchunks;
   cfg->nwork = (int)(cols * nchunks);
   cfg->per_xcd = (cfg->nwork + 7) / 8;
   cfg->grid = 8 * cfg->per_xcd;
-  cfg->lds_bytes = (size_t)ks * best_nt * 64 * sizeof(v4i) + (size_t)best_nt * 32 * (sizeof(int) * 2 + sizeof(float)) +
-                   (size_t)kWaves * 64 * sizeof(v4i);
-  cfg->dish_tail = (int64_t)ks * kDishesPerStep != s.ndishes;
-  cfg->kernel = lookup_mfma(best_nt, ks, cfg->dish_tail);
-  return cfg->kernel != nullptr;
+  return true;
+}
+
+// Resident workgroups per CU of an MFMA kernel configuration.
+int wgs_per_cu(const MfmaConfig &cfg) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(cfg.kernel), kThreads,
+                                                   cfg.lds_bytes) != hipSuccess ||
+      n < 1)
+    n = 1;
+  return n;
+}
+
+// Chooses the MFMA configuration for a shape (tuned choice if one is known).
+// Returns false if the shape is outside the MFMA kernel's range.
+bool choose_mfma_config(const bf_shape &s, int num_cus, MfmaConfig *cfg, const TuneChoice *force) {
+  if (s.npolarizations != 2 || s.ndishes % 8 != 0 || s.ndishes > 1024) return false;
+  TuneChoice tc;
+  const bool tuned = force ? (tc = *force, true) : recall_tuning(s, &tc);
+  if (!choose_tiling(s.ndishes, s.nbeams, tuned ? tc.nt : 0, cfg)) return false;
+  if (hipFuncSetAttribute(reinterpret_cast<const void *>(cfg->kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)cfg->lds_bytes) != hipSuccess)
+    return false;
+  const double scale = tuned ? tc.wg_scale : 1.0;
+  const int64_t target = (int64_t)std::llround(num_cus * wgs_per_cu(*cfg) * scale);
+  return choose_grid(s.ntimes, s.nfrequencies, target, cfg);
 }
 
 static int num_cus_of_current_device(int *dev, int *ncu) {
@@ -125,7 +197,7 @@ static int num_cus_of_current_device(int *dev, int *ncu) {
 }
 
 // (Re)builds the device-side weight data of a plan from device A and G.
-static int prepare_weights(bf_plan *p, hipStream_t st) {
+int prepare_weights(bf_plan *p, hipStream_t st) {
   const bf_shape &s = p->shape;
   const int F = (int)s.nfrequencies, B = (int)s.nbeams, D = (int)s.ndishes;
   if (p->use_mfma) {
@@ -187,7 +259,8 @@ static int plan_create(const bf_shape *shape, const uint8_t *A, const float *G, 
     Gcheck = Gh.data();
   }
   p->fast_gains = gains_fast_safe(Gcheck, nG, s.ndishes);
-  p->use_mfma = p->fast_gains && choose_mfma_config(s, p->num_cus, &p->cfg);
+  p->use_mfma = p->fast_gains && choose_mfma_config(s, p->num_cus, &p->cfg, nullptr);
+  if (p->use_mfma) p->wgs_per_cu = wgs_per_cu(p->cfg);
 
   hipError_t e;
   if ((e = hipMalloc(&p->A_dev, nA)) != hipSuccess) return bail(hip_fail(e, "hipMalloc A"));
@@ -195,21 +268,55 @@ static int plan_create(const bf_shape *shape, const uint8_t *A, const float *G, 
   const hipMemcpyKind kind = host_inputs ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
   if ((e = hipMemcpyAsync(p->A_dev, A, nA, kind, st)) != hipSuccess) return bail(hip_fail(e, "copy A"));
   if ((e = hipMemcpyAsync(p->G_dev, G, nG * sizeof(float), kind, st)) != hipSuccess) return bail(hip_fail(e, "copy G"));
-  if (p->use_mfma) {
-    const MfmaConfig &c = p->cfg;
-    const size_t wbytes = (size_t)s.nfrequencies * c.ks * c.ntp * 64 * sizeof(v4i);
-    const size_t BP = (size_t)c.ntp * kBeamTile;
-    if ((e = hipMalloc(&p->W, wbytes)) != hipSuccess) return bail(hip_fail(e, "hipMalloc W"));
-    if ((e = hipMalloc(&p->corr, s.nfrequencies * BP * 2 * sizeof(int))) != hipSuccess)
-      return bail(hip_fail(e, "hipMalloc corr"));
-    if ((e = hipMalloc(&p->Gp, s.nfrequencies * BP * sizeof(float))) != hipSuccess)
-      return bail(hip_fail(e, "hipMalloc Gp"));
-    if ((e = hipFuncSetAttribute(reinterpret_cast<const void *>(c.kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)c.lds_bytes)) != hipSuccess)
-      return bail(hip_fail(e, "hipFuncSetAttribute(MaxDynamicSharedMemorySize)"));
-  }
+  if (p->use_mfma && (rc = alloc_weight_tables(p))) return bail(rc);
   if ((rc = prepare_weights(p, st))) return bail(rc);
   *out = p;
+  return BF_OK;
+}
+
+// Allocates the weight-operand tables of p->cfg (freeing older ones).
+int alloc_weight_tables(bf_plan *p) {
+  const bf_shape &s = p->shape;
+  const MfmaConfig &c = p->cfg;
+  if (p->W) (void)hipFree(p->W);
+  if (p->corr) (void)hipFree(p->corr);
+  if (p->Gp) (void)hipFree(p->Gp);
+  p->W = nullptr;
+  p->corr = nullptr;
+  p->Gp = nullptr;
+  const size_t wbytes = (size_t)s.nfrequencies * c.ks * c.ntp * 64 * sizeof(v4i);
+  const size_t BP = (size_t)c.ntp * kBeamTile;
+  BF_HIP(hipMalloc(&p->W, wbytes));
+  BF_HIP(hipMalloc(&p->corr, s.nfrequencies * BP * 2 * sizeof(int)));
+  BF_HIP(hipMalloc(&p->Gp, s.nfrequencies * BP * sizeof(float)));
+  return BF_OK;
+}
+
+// Launches the MFMA kernel of cfg over E/J with T times and F frequencies
+// whose weights are rows wrow0 .. wrow0 + F - 1 of the plan's tables.
+int launch_mfma(const bf_plan *p, const MfmaConfig &c, uint8_t *J, const uint8_t *E, int64_t T, int F, int wrow0,
+                hipStream_t st) {
+  const bf_shape &s = p->shape;
+  MfmaParams a;
+  a.E = E;
+  a.J = J;
+  a.W = p->W;
+  a.corr = p->corr;
+  a.G = p->Gp;
+  a.T = T;
+  a.F = F;
+  a.D = (int32_t)s.ndishes;
+  a.B = (int32_t)s.nbeams;
+  a.NTP = c.ntp;
+  a.nbg = c.nbg;
+  a.nchunks = c.nchunks;
+  a.tiles_per_chunk = c.tiles_per_chunk;
+  a.nwork = c.nwork;
+  a.per_xcd = c.per_xcd;
+  a.jlayout = s.j_layout;
+  a.wrow0 = wrow0;
+  void *args[] = {&a};
+  BF_HIP(hipLaunchKernel(reinterpret_cast<const void *>(c.kernel), dim3(c.grid), dim3(kThreads), args, c.lds_bytes, st));
   return BF_OK;
 }
 
@@ -218,27 +325,7 @@ int launch_form_beams(const bf_plan *p, uint8_t *J, const uint8_t *E, hipStream_
   // The MFMA kernel reads E in 16-byte vectors and writes J in 4/8-byte words.
   const bool aligned = ((uintptr_t)E & 15u) == 0 && ((uintptr_t)J & 7u) == 0;
   if (p->use_mfma && aligned) {
-    const MfmaConfig &c = p->cfg;
-    MfmaParams a;
-    a.E = E;
-    a.J = J;
-    a.W = p->W;
-    a.corr = p->corr;
-    a.G = p->Gp;
-    a.T = s.ntimes;
-    a.F = (int32_t)s.nfrequencies;
-    a.D = (int32_t)s.ndishes;
-    a.B = (int32_t)s.nbeams;
-    a.NTP = c.ntp;
-    a.nbg = c.nbg;
-    a.nchunks = c.nchunks;
-    a.tiles_per_chunk = c.tiles_per_chunk;
-    a.nwork = c.nwork;
-    a.per_xcd = c.per_xcd;
-    a.jlayout = s.j_layout;
-    void *args[] = {&a};
-    BF_HIP(hipLaunchKernel(reinterpret_cast<const void *>(c.kernel), dim3(c.grid), dim3(kThreads), args, c.lds_bytes,
-                           st));
+    return launch_mfma(p, p->cfg, J, E, s.ntimes, (int)s.nfrequencies, 0, st);
   } else {
     const int64_t total = s.nbeams * s.nfrequencies * s.npolarizations * s.ntimes;
     const int blocks = (int)std::min<int64_t>((total + 255) / 256, 1 << 20);

@@ -1,0 +1,397 @@
+// bf_batch.hip -- one-channel launches, the batched persistent launch with
+// its device-side work queue and hipGraph replay, and the autotuners.
+#include "beamform.h"
+#include "bf_plan.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <functional>
+#include <string>
+#include <vector>
+
+using namespace bf;
+
+#define BF_HIP(call)                                                                                                   \
+  do {                                                                                                                 \
+    const hipError_t e_ = (call);                                                                                      \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);                                                                  \
+  } while (0)
+
+struct bf_batch {
+  const bf_plan *plan = nullptr;
+  int max_chunks = 0;
+  int item_capacity = 0;
+  int item_tiles = 0; // 0 = automatic
+  int last_item_tiles = 0;
+  int last_nitems = 0;
+  int grid = 0;
+  BatchChunk *d_chunks = nullptr;
+  BatchItem *d_items = nullptr;
+  int *d_nitems = nullptr;
+  unsigned *d_head = nullptr;
+  BatchChunk *h_chunks = nullptr; // pinned staging
+  BatchItem *h_items = nullptr;   // pinned staging
+  int *h_nitems = nullptr;        // pinned staging
+  hipEvent_t upload_done = nullptr;
+  bool upload_pending = false;
+  hipStream_t capture_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int check_channel_args(const bf_plan *p, int channel, int64_t ntimes, const void *J, const void *E) {
+  if (!p || !J || !E) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (channel < 0 || channel >= p->shape.nfrequencies) return fail(BF_ERR_INVALID_ARGUMENT, "channel out of range");
+  if (ntimes < 1) return fail(BF_ERR_INVALID_ARGUMENT, "ntimes must be >= 1");
+  return BF_OK;
+}
+
+void free_graph(bf_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+int ensure_item_capacity(bf_batch *b, int n) {
+  if (n <= b->item_capacity) return BF_OK;
+  int cap = std::max(n, 2 * b->item_capacity);
+  if (b->upload_pending) {
+    BF_HIP(hipEventSynchronize(b->upload_done));
+    b->upload_pending = false;
+  }
+  if (b->d_items) (void)hipFree(b->d_items);
+  if (b->h_items) (void)hipHostFree(b->h_items);
+  b->d_items = nullptr;
+  b->h_items = nullptr;
+  b->item_capacity = 0;
+  BF_HIP(hipMalloc(&b->d_items, (size_t)cap * sizeof(BatchItem)));
+  BF_HIP(hipHostMalloc(&b->h_items, (size_t)cap * sizeof(BatchItem), hipHostMallocDefault));
+  b->item_capacity = cap;
+  free_graph(b); // the captured launch refers to the old item buffer
+  return BF_OK;
+}
+
+// Default item size: about two items per resident workgroup, a multiple of
+// the 8 waves, between 8 and 256 tiles.
+int default_item_tiles(int64_t total_tiles, int grid) {
+  int64_t t = (total_tiles + 2 * (int64_t)grid - 1) / (2 * (int64_t)grid);
+  t = (t + kWaves - 1) / kWaves * kWaves;
+  return (int)std::min<int64_t>(256, std::max<int64_t>(kWaves, t));
+}
+
+// Validates the chunk list, builds the work items (longest first) into the
+// pinned staging buffers and copies both to the device on `st`.
+int upload(bf_batch *b, const bf_chunk *chunks, int n, hipStream_t st) {
+  const bf_plan *p = b->plan;
+  if (n < 0 || n > b->max_chunks) return fail(BF_ERR_INVALID_ARGUMENT, "nchunks outside [0, max_chunks]");
+  if (n > 0 && !chunks) return fail(BF_ERR_INVALID_ARGUMENT, "chunks is NULL");
+  const MfmaConfig &c = p->cfg;
+  int64_t total_tiles = 0;
+  for (int i = 0; i < n; ++i) {
+    const bf_chunk &k = chunks[i];
+    if (!k.E || !k.J || k.ntimes < 1 || k.reserved != 0 || k.channel < 0 || k.channel >= p->shape.nfrequencies)
+      return fail(BF_ERR_INVALID_ARGUMENT, "bad chunk " + std::to_string(i));
+    if (((uintptr_t)k.E & 15u) || ((uintptr_t)k.J & 7u))
+      return fail(BF_ERR_INVALID_ARGUMENT, "chunk " + std::to_string(i) + ": E must be 16-byte and J 8-byte aligned");
+    total_tiles += (k.ntimes + kTileT - 1) / kTileT * c.nbg;
+  }
+  const int it = b->item_tiles > 0 ? b->item_tiles : default_item_tiles(total_tiles, b->grid);
+  std::vector<BatchItem> items;
+  items.reserve((size_t)(total_tiles / it + (int64_t)n * c.nbg + 1));
+  for (int i = 0; i < n; ++i) {
+    const int64_t nt = (chunks[i].ntimes + kTileT - 1) / kTileT;
+    if (nt > (1ll << 30)) return fail(BF_ERR_UNSUPPORTED, "chunk too long");
+    for (int bg = 0; bg < c.nbg; ++bg)
+      for (int64_t t = 0; t < nt; t += it) items.push_back(BatchItem{i, bg, (int32_t)t, (int32_t)std::min(nt, t + it)});
+  }
+  std::stable_sort(items.begin(), items.end(), [](const BatchItem &x, const BatchItem &y) {
+    return x.tile_end - x.tile_begin > y.tile_end - y.tile_begin;
+  });
+  if (int rc = ensure_item_capacity(b, std::max<int>(1, (int)items.size()))) return rc;
+  if (b->upload_pending) { // the staging buffers may still be read by the previous copy
+    BF_HIP(hipEventSynchronize(b->upload_done));
+    b->upload_pending = false;
+  }
+  for (int i = 0; i < n; ++i)
+    b->h_chunks[i] = BatchChunk{chunks[i].E, chunks[i].J, chunks[i].ntimes, chunks[i].channel, 0};
+  std::copy(items.begin(), items.end(), b->h_items);
+  *b->h_nitems = (int)items.size();
+  if (n > 0) BF_HIP(hipMemcpyAsync(b->d_chunks, b->h_chunks, (size_t)n * sizeof(BatchChunk), hipMemcpyHostToDevice, st));
+  if (!items.empty())
+    BF_HIP(hipMemcpyAsync(b->d_items, b->h_items, items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, st));
+  BF_HIP(hipMemcpyAsync(b->d_nitems, b->h_nitems, sizeof(int), hipMemcpyHostToDevice, st));
+  BF_HIP(hipEventRecord(b->upload_done, st));
+  b->upload_pending = true;
+  b->last_item_tiles = it;
+  b->last_nitems = (int)items.size();
+  return BF_OK;
+}
+
+BatchParams batch_params(const bf_batch *b) {
+  const bf_plan *p = b->plan;
+  BatchParams a;
+  a.chunks = b->d_chunks;
+  a.items = b->d_items;
+  a.nitems = b->d_nitems;
+  a.head = b->d_head;
+  a.W = p->W;
+  a.corr = p->corr;
+  a.G = p->Gp;
+  a.NTP = p->cfg.ntp;
+  a.D = (int32_t)p->shape.ndishes;
+  a.B = (int32_t)p->shape.nbeams;
+  a.jlayout = p->shape.j_layout;
+  return a;
+}
+
+// The launch itself: queue reset, then the persistent kernel.
+int enqueue_launch(bf_batch *b, hipStream_t st) {
+  const MfmaConfig &c = b->plan->cfg;
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, st, b->d_head);
+  BF_HIP(hipGetLastError());
+  BatchParams a = batch_params(b);
+  void *args[] = {&a};
+  BF_HIP(hipLaunchKernel(reinterpret_cast<const void *>(c.batch_kernel), dim3(b->grid), dim3(kThreads), args,
+                         c.lds_bytes, st));
+  return BF_OK;
+}
+
+int time_launches(hipStream_t st, int iters, float *ms, const std::function<int()> &launch) {
+  hipEvent_t e0, e1;
+  BF_HIP(hipEventCreate(&e0));
+  BF_HIP(hipEventCreate(&e1));
+  int rc = launch(); // warm-up
+  if (!rc) {
+    const hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = hip_fail(e, "warm-up launch");
+  }
+  if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = BF_ERR_DEVICE;
+  for (int i = 0; !rc && i < iters; ++i) rc = launch();
+  if (!rc && hipEventRecord(e1, st) != hipSuccess) rc = BF_ERR_DEVICE;
+  if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = BF_ERR_DEVICE;
+  float t = 0;
+  if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = BF_ERR_DEVICE;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = iters > 0 ? t / iters : t;
+  return rc;
+}
+
+} // namespace
+
+extern "C" {
+
+int bf_form_beams_channel(const bf_plan *p, int channel, int64_t ntimes, uint8_t *J, const uint8_t *E, void *stream) {
+  if (int rc = check_channel_args(p, channel, ntimes, J, E)) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const bf_shape &s = p->shape;
+  const bool aligned = ((uintptr_t)E & 15u) == 0 && ((uintptr_t)J & 7u) == 0;
+  if (p->use_mfma && aligned) {
+    MfmaConfig c = p->cfg;
+    if (!choose_grid(ntimes, 1, (int64_t)p->num_cus * p->wgs_per_cu, &c))
+      return fail(BF_ERR_UNSUPPORTED, "ntimes too large");
+    return launch_mfma(p, c, J, E, ntimes, 1, channel, st);
+  }
+  const int64_t total = s.nbeams * s.npolarizations * ntimes;
+  const int blocks = (int)std::min<int64_t>((total + 255) / 256, 1 << 20);
+  hipLaunchKernelGGL(form_beams_generic_kernel, dim3(blocks), dim3(256), 0, st, E, J,
+                     p->A_dev + (int64_t)channel * s.nbeams * s.ndishes, p->G_dev + (int64_t)channel * s.nbeams,
+                     ntimes, 1, (int)s.ndishes, (int)s.npolarizations, (int)s.nbeams, s.j_layout);
+  BF_HIP(hipGetLastError());
+  return BF_OK;
+}
+
+int bf_batch_create(const bf_plan *p, int max_chunks, bf_batch **out) {
+  if (!out) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  if (!p || max_chunks < 1) return fail(BF_ERR_INVALID_ARGUMENT, "NULL plan or max_chunks < 1");
+  if (!p->use_mfma)
+    return fail(BF_ERR_UNSUPPORTED, "batching needs the MFMA path (P = 2, D % 8 == 0, D <= 1024, gains in range)");
+  bf_batch *b = new bf_batch();
+  b->plan = p;
+  b->max_chunks = max_chunks;
+  auto bail = [&](int rc) {
+    bf_batch_destroy(b);
+    return rc;
+  };
+  const MfmaConfig &c = p->cfg;
+  hipError_t e;
+  if ((e = hipFuncSetAttribute(reinterpret_cast<const void *>(c.batch_kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds_bytes)) != hipSuccess)
+    return bail(hip_fail(e, "hipFuncSetAttribute(batch)"));
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(c.batch_kernel), kThreads,
+                                                   c.lds_bytes) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  b->grid = p->num_cus * per_cu;
+  if ((e = hipMalloc(&b->d_chunks, (size_t)max_chunks * sizeof(BatchChunk))) != hipSuccess)
+    return bail(hip_fail(e, "hipMalloc chunks"));
+  if ((e = hipMalloc(&b->d_nitems, sizeof(int))) != hipSuccess) return bail(hip_fail(e, "hipMalloc nitems"));
+  if ((e = hipMalloc(&b->d_head, sizeof(unsigned))) != hipSuccess) return bail(hip_fail(e, "hipMalloc head"));
+  if ((e = hipHostMalloc(&b->h_chunks, (size_t)max_chunks * sizeof(BatchChunk), hipHostMallocDefault)) != hipSuccess)
+    return bail(hip_fail(e, "hipHostMalloc chunks"));
+  if ((e = hipHostMalloc(&b->h_nitems, sizeof(int), hipHostMallocDefault)) != hipSuccess)
+    return bail(hip_fail(e, "hipHostMalloc nitems"));
+  if ((e = hipEventCreateWithFlags(&b->upload_done, hipEventDisableTiming)) != hipSuccess)
+    return bail(hip_fail(e, "hipEventCreate"));
+  if ((e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking)) != hipSuccess)
+    return bail(hip_fail(e, "hipStreamCreate"));
+  if (int rc = ensure_item_capacity(b, 4 * b->grid)) return bail(rc);
+  *out = b;
+  return BF_OK;
+}
+
+int bf_batch_destroy(bf_batch *b) {
+  if (!b) return BF_OK;
+  if (b->upload_pending) (void)hipEventSynchronize(b->upload_done);
+  free_graph(b);
+  if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+  if (b->upload_done) (void)hipEventDestroy(b->upload_done);
+  if (b->d_chunks) (void)hipFree(b->d_chunks);
+  if (b->d_items) (void)hipFree(b->d_items);
+  if (b->d_nitems) (void)hipFree(b->d_nitems);
+  if (b->d_head) (void)hipFree(b->d_head);
+  if (b->h_chunks) (void)hipHostFree(b->h_chunks);
+  if (b->h_items) (void)hipHostFree(b->h_items);
+  if (b->h_nitems) (void)hipHostFree(b->h_nitems);
+  delete b;
+  return BF_OK;
+}
+
+int bf_batch_set_item_tiles(bf_batch *b, int item_tiles) {
+  if (!b || item_tiles < 0) return fail(BF_ERR_INVALID_ARGUMENT, "bad argument");
+  b->item_tiles = item_tiles == 0 ? 0 : std::max(kWaves, (item_tiles + kWaves - 1) / kWaves * kWaves);
+  return BF_OK;
+}
+
+int bf_batch_run(bf_batch *b, const bf_chunk *chunks, int nchunks, void *stream) {
+  if (!b) return fail(BF_ERR_INVALID_ARGUMENT, "NULL batch");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (int rc = upload(b, chunks, nchunks, st)) return rc;
+  return enqueue_launch(b, st);
+}
+
+int bf_batch_run_graph(bf_batch *b, const bf_chunk *chunks, int nchunks, void *stream) {
+  if (!b) return fail(BF_ERR_INVALID_ARGUMENT, "NULL batch");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (int rc = upload(b, chunks, nchunks, st)) return rc;
+  if (!b->exec) {
+    // Capture once: the launch reads chunks, items and their count from the
+    // batch's device buffers, so the graph stays valid for any chunk list.
+    BF_HIP(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_launch(b, b->capture_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->capture_stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    b->graph = g;
+    BF_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  BF_HIP(hipGraphLaunch(b->exec, st));
+  return BF_OK;
+}
+
+int bf_batch_describe(const bf_batch *b, char *buf, size_t len) {
+  if (!b || !buf || len == 0) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  const MfmaConfig &c = b->plan->cfg;
+  std::snprintf(buf, len,
+                "persistent mfma_i32_32x32x32_i8 batch: nt=%d ks=%d nbg=%d grid=%d lds=%zu item_tiles=%d%s items=%d "
+                "graph=%s",
+                c.nt, c.ks, c.nbg, b->grid, c.lds_bytes, b->last_item_tiles, b->item_tiles ? "" : " (auto)",
+                b->last_nitems, b->exec ? "captured" : "none");
+  return BF_OK;
+}
+
+int bf_plan_autotune(bf_plan *p, uint8_t *J, const uint8_t *E, void *stream, int iters, char *report,
+                     size_t report_len) {
+  if (!p || !J || !E) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!p->use_mfma) return fail(BF_ERR_UNSUPPORTED, "the plan does not use the MFMA kernel");
+  if (iters <= 0) iters = 10;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const bf_shape s = p->shape;
+  std::string rep;
+  TuneChoice best;
+  float best_ms = 1e30f;
+  int min_ntp = 1 << 30;
+  for (int nt = 1; nt <= 4; ++nt) {
+    MfmaConfig t;
+    if (choose_tiling(s.ndishes, s.nbeams, nt, &t)) min_ntp = std::min(min_ntp, t.ntp);
+  }
+  for (int nt = 1; nt <= 4; ++nt) {
+    MfmaConfig t;
+    if (!choose_tiling(s.ndishes, s.nbeams, nt, &t)) continue;
+    if (3 * t.ntp > 4 * min_ntp) continue; // pads a third more beam tiles than the best tiling
+    for (double scale : {0.5, 1.0, 2.0}) {
+      const TuneChoice choice{nt, scale};
+      MfmaConfig c;
+      if (!choose_mfma_config(s, p->num_cus, &c, &choice)) continue;
+      p->cfg = c;
+      if (int rc = alloc_weight_tables(p)) return rc;
+      if (int rc = prepare_weights(p, st)) return rc;
+      float ms = 0;
+      if (int rc = time_launches(st, iters, &ms, [&] { return launch_form_beams(p, J, E, st); })) return rc;
+      char line[160];
+      std::snprintf(line, sizeof line, "nt=%d nbg=%d scale=%.1f grid=%d: %.1f us\n", c.nt, c.nbg, scale, c.grid,
+                    ms * 1e3f);
+      rep += line;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = choice;
+      }
+    }
+  }
+  MfmaConfig c;
+  if (!choose_mfma_config(s, p->num_cus, &c, &best)) return fail(BF_ERR_UNSUPPORTED, "no candidate configuration");
+  p->cfg = c;
+  p->wgs_per_cu = wgs_per_cu(c);
+  if (int rc = alloc_weight_tables(p)) return rc;
+  if (int rc = prepare_weights(p, st)) return rc;
+  remember_tuning(s, best);
+  char line[160];
+  std::snprintf(line, sizeof line, "chosen: nt=%d scale=%.1f (%.1f us)\n", best.nt, best.wg_scale, best_ms * 1e3f);
+  rep += line;
+  if (report && report_len) std::snprintf(report, report_len, "%s", rep.c_str());
+  return BF_OK;
+}
+
+int bf_batch_autotune(bf_batch *b, const bf_chunk *chunks, int nchunks, void *stream, int iters, char *report,
+                      size_t report_len) {
+  if (!b) return fail(BF_ERR_INVALID_ARGUMENT, "NULL batch");
+  if (iters <= 0) iters = 5;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  std::string rep;
+  int best_tiles = 0;
+  float best_ms = 1e30f;
+  const int saved = b->item_tiles;
+  for (int tiles : {0, 16, 32, 64, 128, 256}) {
+    b->item_tiles = tiles;
+    float ms = 0;
+    if (int rc = time_launches(st, iters, &ms, [&] { return bf_batch_run(b, chunks, nchunks, stream); })) {
+      b->item_tiles = saved;
+      return rc;
+    }
+    char line[128];
+    std::snprintf(line, sizeof line, "item_tiles=%d%s: %.1f us\n", b->last_item_tiles, tiles ? "" : " (auto)",
+                  ms * 1e3f);
+    rep += line;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best_tiles = tiles;
+    }
+  }
+  b->item_tiles = best_tiles;
+  char line[96];
+  std::snprintf(line, sizeof line, "chosen: item_tiles=%d%s\n", best_tiles, best_tiles ? "" : " (auto)");
+  rep += line;
+  if (report && report_len) std::snprintf(report, report_len, "%s", rep.c_str());
+  return BF_OK;
+}
+
+} // extern "C"

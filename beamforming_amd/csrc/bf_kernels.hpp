@@ -23,5 +23,11 @@ MfmaKernel mfma_kernel_ks8(int nt, bool dish_tail);
 MfmaKernel mfma_kernel_ks16(int nt, bool dish_tail);
 MfmaKernel mfma_kernel_ks32(int nt, bool dish_tail);
 MfmaKernel mfma_kernel_ks64(int nt, bool dish_tail);
+BatchKernel batch_kernel_ks2(int nt, bool dish_tail);
+BatchKernel batch_kernel_ks4(int nt, bool dish_tail);
+BatchKernel batch_kernel_ks8(int nt, bool dish_tail);
+BatchKernel batch_kernel_ks16(int nt, bool dish_tail);
+BatchKernel batch_kernel_ks32(int nt, bool dish_tail);
+BatchKernel batch_kernel_ks64(int nt, bool dish_tail);
 
 } // namespace bf

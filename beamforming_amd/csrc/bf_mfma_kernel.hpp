@@ -68,54 +68,67 @@ __device__ __forceinline__ void store_row8(uint8_t *dst, uint32_t w0, uint32_t w
   }
 }
 
+// LDS of a workgroup: [KS][NT][64] weight operands, the beam-column nibble
+// offsets and gains of the beam group, then one 1 KiB E transpose buffer per
+// wave.
+template <int NT, int KS> struct LdsPlan {
+  static constexpr size_t weights = (size_t)KS * NT * 64 * sizeof(v4i);
+  static constexpr size_t beams = (size_t)NT * 32 * (sizeof(int2) + sizeof(float));
+  static constexpr size_t transpose = (size_t)kWaves * 64 * sizeof(v4i);
+  static constexpr size_t bytes = weights + beams + transpose;
+};
+
+// Stages row `wrow` (a frequency, or a channel of a channel table) of the
+// prepared weights, beam group bg, into LDS. All threads of the workgroup
+// take part; the caller synchronises.
+template <int NT, int KS>
+__device__ __forceinline__ void stage_weights(const v4i *W, const int *corr, const float *G, int NTP, int wrow, int bg,
+                                              v4i *Wl, int2 *Cl, float *Gl) {
+  const v4i *src = W + ((size_t)wrow * KS * NTP + (size_t)bg * NT) * 64;
+  for (int i = threadIdx.x; i < KS * NT * 64; i += kThreads) {
+    const int s = i / (NT * 64);
+    const int rem = i - s * (NT * 64);
+    Wl[i] = src[(size_t)s * NTP * 64 + rem];
+  }
+  const size_t g0 = (size_t)wrow * NTP * 32 + (size_t)bg * NT * 32;
+  for (int i = threadIdx.x; i < NT * 32; i += kThreads) {
+    Cl[i] = make_int2(corr[(g0 + i) * 2 + 0], corr[(g0 + i) * 2 + 1]);
+    Gl[i] = G[g0 + i];
+  }
+}
+
+// One stream of E rows (one frequency of one chunk) and where its J goes.
+struct TileJob {
+  const uint8_t *E;  // E byte of (t = 0, d = 0, p = 0) of this frequency
+  int64_t e_tstride; // bytes between consecutive times (F D 2)
+  uint8_t *J;        // J of the whole [B][F][..] array
+  int64_t T;         // time samples of the stream
+  int32_t F, f;      // frequencies in J, and this stream's frequency
+  int32_t D, B;      // dishes, beams
+  int32_t bg;        // beam group (NT 32-beam tiles)
+  int32_t jlayout;   // 0 = [b][f][p][t], 1 = [b][f][t][p]
+};
+
+// The per-wave pipeline: tiles tile, tile + step, ... < tile_end of one job,
+// against the weights staged in LDS (Wl, Cl, Gl), using the wave's transpose
+// buffer xw.
 template <int NT, int KS, bool DT>
-__global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) {
+__device__ __forceinline__ void run_tiles(const TileJob &job, int64_t tile, const int64_t tile_end, const int64_t step,
+                                          const v4i *Wl, const int2 *Cl, const float *Gl, v4i *xw) {
   static_assert(NT >= 1 && NT <= 4, "32-beam tiles per workgroup");
   static_assert(KS >= 2 && KS <= 64 && KS % 2 == 0, "K steps (pairs)");
   constexpr int KP = KS / 2;                  // K-step pairs (one E load each)
   constexpr int RP0 = NT >= 4 ? 8 : 16;       // E prefetch distance in pairs
   constexpr int RP = RP0 < KP ? RP0 : KP;
   static_assert(KP % RP == 0, "ring slots must line up across tiles");
-  extern __shared__ v4i Wl[]; // [KS][NT][64] weights, then per-beam data
-  // Per beam column of the group: nibble offsets and gains, after the weights.
-  int2 *Cl = reinterpret_cast<int2 *>(Wl + KS * NT * 64); // [NT * 32]
-  float *Gl = reinterpret_cast<float *>(Cl + NT * 32);     // [NT * 32]
-  // Per-wave E transpose buffers: 16 rows x 64 bytes each.
-  v4i *Xl = reinterpret_cast<v4i *>(Gl + NT * 32);        // [kWaves][16][4]
-
-  const int work = work_of_block(blockIdx.x, a.per_xcd);
-  if (work >= a.nwork) return; // uniform per workgroup, before any barrier
-  const WorkItem wi = decode_work(work, a.nbg, a.nchunks);
-  const int f = wi.f;
-
-  {
-    const v4i *src = a.W + ((size_t)f * KS * a.NTP + (size_t)wi.bg * NT) * 64;
-    for (int i = threadIdx.x; i < KS * NT * 64; i += kThreads) {
-      const int s = i / (NT * 64);
-      const int rem = i - s * (NT * 64);
-      Wl[i] = src[(size_t)s * a.NTP * 64 + rem];
-    }
-    const size_t g0 = (size_t)f * a.NTP * 32 + (size_t)wi.bg * NT * 32;
-    for (int i = threadIdx.x; i < NT * 32; i += kThreads) {
-      Cl[i] = make_int2(a.corr[(g0 + i) * 2 + 0], a.corr[(g0 + i) * 2 + 1]);
-      Gl[i] = a.G[g0 + i];
-    }
-  }
-  __syncthreads();
+  if (tile >= tile_end) return;
 
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
   const int h = lane >> 5;          // k half (operands); row half (accumulators)
   const int col = lane & 31;        // beam column of the accumulators
   const int pm = (lane >> 4) & 1;   // polarization of this lane's E row
   const int tm = lane & 15;         // time of this lane's E row within the tile
   const uint32_t sel = pm ? 0x07050301u : 0x06040200u;
-
-  const int64_t ntiles = (a.T + kTileT - 1) / kTileT;
-  const int64_t tile_begin = (int64_t)wi.chunk * a.tiles_per_chunk;
-  const int64_t tile_end = ntiles < tile_begin + a.tiles_per_chunk ? ntiles : tile_begin + a.tiles_per_chunk;
-  int64_t tile = tile_begin + wave;
-  if (tile >= tile_end) return; // no barrier follows
 
   // E loads, fully coalesced. Row t of this frequency starts at
   // E + (t F + f) D 2. For K-step pair (2j, 2j + 1) a tile needs bytes
@@ -129,20 +142,19 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
   // dishes past D read the row's first bytes); out-of-range values are zeroed
   // where they are consumed, never right after the load.
   const int ld_row = lane >> 2, ld_piece = lane & 3;
-  const int64_t e_tstride = (int64_t)a.F * a.D * 2;
-  const uint8_t *Ef = a.E + (int64_t)f * a.D * 2 + ld_piece * 16;
-  const int64_t tlast = a.T - 1;
+  const int64_t e_tstride = job.e_tstride;
+  const uint8_t *Ef = job.E + ld_piece * 16;
+  const int64_t tlast = job.T - 1;
   auto row_ptr = [&](int64_t tl) -> const uint8_t * {
     const int64_t t = tl * kTileT + ld_row;
-    return Ef + (t < a.T ? t : tlast) * e_tstride;
+    return Ef + (t < job.T ? t : tlast) * e_tstride;
   };
   // dishes of K step s for the operand lanes of half h
-  auto dish_ok = [&](int s) -> bool { return kDishesPerStep * s + 8 * h < a.D; };
+  auto dish_ok = [&](int s) -> bool { return kDishesPerStep * s + 8 * h < job.D; };
   auto load_pair = [&](const uint8_t *p, int j) -> v4i {
-    if (DT) return *reinterpret_cast<const v4i *>(p + (32 * j + 8 * ld_piece < a.D ? 64 * j : -16 * ld_piece));
+    if (DT) return *reinterpret_cast<const v4i *>(p + (32 * j + 8 * ld_piece < job.D ? 64 * j : -16 * ld_piece));
     return *reinterpret_cast<const v4i *>(p + 64 * j);
   };
-  v4i *xw = Xl + wave * 64;
   const int wr_slot = ld_row * 4 + ((ld_piece + (ld_row >> 1)) & 3);
   int rd_slot[2];
 #pragma unroll
@@ -155,7 +167,7 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
     for (int q = 0; q < RP; ++q) ring[q] = load_pair(p0, q);
   }
 
-  for (; tile < tile_end; tile += kWaves) {
+  for (; tile < tile_end; tile += step) {
     v16i acc[NT][2];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
@@ -166,9 +178,9 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
         acc[nt][1][j] = -c.y;
       }
     }
-    const bool row_ok = tile * kTileT + tm < a.T; // row of this lane's operand
+    const bool row_ok = tile * kTileT + tm < job.T; // row of this lane's operand
     const uint8_t *pcur = row_ptr(tile);
-    const uint8_t *pnext = row_ptr(tile + kWaves);
+    const uint8_t *pnext = row_ptr(tile + step);
 
     // One K loop in straight-line code (a scheduling region per step): the
     // ring of loads in flight never crosses a loop back-edge inside a tile,
@@ -223,7 +235,7 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
         v2c = v2n;
       }
     };
-    if (DT || (tile + 1) * kTileT > a.T) // wave-uniform
+    if (DT || (tile + 1) * kTileT > job.T) // wave-uniform
       kloop(std::true_type{});
     else
       kloop(std::false_type{});
@@ -232,7 +244,7 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
     // 8 (j / 4) + 4 h + j % 4 = (polarization j / 8, time 8 ((j / 4) & 1) +
     // 4 h + j % 4) of beam column col.
     const int64_t t0 = tile * kTileT;
-    const int nvalid = a.T - t0 < kTileT ? (int)(a.T - t0) : kTileT;
+    const int nvalid = job.T - t0 < kTileT ? (int)(job.T - t0) : kTileT;
     uint32_t wv[NT][4]; // [p0: t 4h.., p0: t 8+4h.., p1: t 4h.., p1: t 8+4h..]
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
@@ -256,24 +268,24 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
     __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0)
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
-      const int b = (wi.bg * NT + nt) * 32 + col;
-      if (a.jlayout == 0) {
+      const int b = (job.bg * NT + nt) * 32 + col;
+      if (job.jlayout == 0) {
         // J[b][f][p][t]: lane h = 0 assembles polarization 0, lane h = 1
         // polarization 1, exchanging the other halves with lane ^ 32.
         const uint32_t x = h ? wv[nt][0] : wv[nt][2];
         const uint32_t y = h ? wv[nt][1] : wv[nt][3];
         const uint32_t xr = (uint32_t)__shfl_xor((int)x, 32, 64);
         const uint32_t yr = (uint32_t)__shfl_xor((int)y, 32, 64);
-        if (b < a.B) {
-          uint8_t *dst = a.J + (((int64_t)b * a.F + f) * 2 + h) * a.T + t0;
+        if (b < job.B) {
+          uint8_t *dst = job.J + (((int64_t)b * job.F + job.f) * 2 + h) * job.T + t0;
           if (h == 0)
             store_row16(dst, wv[nt][0], xr, wv[nt][1], yr, nvalid);
           else
             store_row16(dst, xr, wv[nt][2], yr, wv[nt][3], nvalid);
         }
-      } else if (b < a.B) {
+      } else if (b < job.B) {
         // J[b][f][t][p]: interleave the polarizations of 4 times per half.
-        uint8_t *row = a.J + (((int64_t)b * a.F + f) * a.T + t0) * 2;
+        uint8_t *row = job.J + (((int64_t)b * job.F + job.f) * job.T + t0) * 2;
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
           const int tt = 8 * k + 4 * h; // first time of this group in the tile
@@ -286,9 +298,109 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
   }
 }
 
-// Kernel table: one instance per (NT, KS, DT), registered by the
+template <int NT, int KS, bool DT>
+__global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) {
+  extern __shared__ v4i Wl[];
+  int2 *Cl = reinterpret_cast<int2 *>(Wl + KS * NT * 64);
+  float *Gl = reinterpret_cast<float *>(Cl + NT * 32);
+  v4i *Xl = reinterpret_cast<v4i *>(Gl + NT * 32);
+
+  const int work = work_of_block(blockIdx.x, a.per_xcd);
+  if (work >= a.nwork) return; // uniform per workgroup, before any barrier
+  const WorkItem wi = decode_work(work, a.nbg, a.nchunks);
+  stage_weights<NT, KS>(a.W, a.corr, a.G, a.NTP, a.wrow0 + wi.f, wi.bg, Wl, Cl, Gl);
+  __syncthreads();
+
+  const int wave = threadIdx.x >> 6;
+  const int64_t ntiles = (a.T + kTileT - 1) / kTileT;
+  const int64_t tile_begin = (int64_t)wi.chunk * a.tiles_per_chunk;
+  const int64_t tile_end = ntiles < tile_begin + a.tiles_per_chunk ? ntiles : tile_begin + a.tiles_per_chunk;
+  TileJob job;
+  job.E = a.E + (int64_t)wi.f * a.D * 2;
+  job.e_tstride = (int64_t)a.F * a.D * 2;
+  job.J = a.J;
+  job.T = a.T;
+  job.F = a.F;
+  job.f = wi.f;
+  job.D = a.D;
+  job.B = a.B;
+  job.bg = wi.bg;
+  job.jlayout = a.jlayout;
+  run_tiles<NT, KS, DT>(job, tile_begin + wave, tile_end, kWaves, Wl, Cl, Gl, Xl + wave * 64);
+}
+
+// ---------------------------------------------------------------------------
+// Batched form_beams: independent chunks (one channel each, own T) in one
+// persistent launch. The host builds a list of work items (chunk, beam group,
+// tile range), longest first; workgroups take items from a device-side queue
+// (an atomic head counter) until it is empty, restaging the weights in LDS
+// only when the (channel, beam group) changes.
+struct BatchChunk {
+  const uint8_t *E; // [T][1][D][2]
+  uint8_t *J;       // [B][1][2][T] or [B][1][T][2]
+  int64_t T;
+  int32_t channel;
+  int32_t pad;
+};
+struct BatchItem {
+  int32_t chunk, bg, tile_begin, tile_end;
+};
+struct BatchParams {
+  const BatchChunk *chunks;
+  const BatchItem *items;
+  const int *nitems;  // device: number of valid items
+  unsigned *head;     // device: queue head, zeroed before each launch
+  const v4i *W;       // channel table [C][KS][NTP][64]
+  const int *corr;    // [C][BP][2]
+  const float *G;     // [C][BP]
+  int32_t NTP, D, B, jlayout;
+};
+
+template <int NT, int KS, bool DT>
+__global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a) {
+  extern __shared__ v4i Wl[];
+  int2 *Cl = reinterpret_cast<int2 *>(Wl + KS * NT * 64);
+  float *Gl = reinterpret_cast<float *>(Cl + NT * 32);
+  v4i *Xl = reinterpret_cast<v4i *>(Gl + NT * 32);
+  __shared__ int s_item;
+  const int wave = threadIdx.x >> 6;
+  int loaded_channel = -1, loaded_bg = -1;
+  const int nitems = *a.nitems;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(a.head, 1u);
+    __syncthreads();
+    const int it = s_item;
+    if (it >= nitems) break; // uniform: every thread read the same s_item
+    const BatchItem item = a.items[it];
+    const BatchChunk ch = a.chunks[item.chunk];
+    if (ch.channel != loaded_channel || item.bg != loaded_bg) {
+      stage_weights<NT, KS>(a.W, a.corr, a.G, a.NTP, ch.channel, item.bg, Wl, Cl, Gl);
+      loaded_channel = ch.channel;
+      loaded_bg = item.bg;
+    }
+    __syncthreads();
+    TileJob job;
+    job.E = ch.E;
+    job.e_tstride = (int64_t)a.D * 2;
+    job.J = ch.J;
+    job.T = ch.T;
+    job.F = 1;
+    job.f = 0;
+    job.D = a.D;
+    job.B = a.B;
+    job.bg = item.bg;
+    job.jlayout = a.jlayout;
+    run_tiles<NT, KS, DT>(job, item.tile_begin + wave, item.tile_end, kWaves, Wl, Cl, Gl, Xl + wave * 64);
+    __syncthreads(); // LDS and s_item are reused by the next item
+  }
+}
+
+__global__ void batch_reset_kernel(unsigned *head);
+
+// Kernel tables: one instance per (NT, KS, DT), registered by the
 // bf_mfma_ks*.hip translation units. DT = true handles ndishes that are not
 // KS * 16 (the last K steps are partly or wholly past D).
 typedef void (*MfmaKernel)(const MfmaParams);
+typedef void (*BatchKernel)(const BatchParams);
 
 } // namespace bf

@@ -22,6 +22,14 @@ struct MfmaConfig {
   int grid = 0;            // launched workgroups (8 * per_xcd)
   size_t lds_bytes = 0;    // dynamic LDS per workgroup
   MfmaKernel kernel = nullptr;
+  BatchKernel batch_kernel = nullptr;
+};
+
+// A tuned choice for one shape: beam tiles per workgroup and a scale on the
+// default number of workgroups.
+struct TuneChoice {
+  int nt = 0;
+  double wg_scale = 1.0;
 };
 
 } // namespace bf
@@ -33,6 +41,7 @@ struct bf_plan {
   bool fast_gains = false; // all gains inside the fast requantisation range
   bool use_mfma = false;   // MFMA kernel (else the generic kernel)
   bf::MfmaConfig cfg;
+  int wgs_per_cu = 1;      // resident MFMA workgroups per CU
   uint8_t *A_dev = nullptr; // [F][B][D] as given
   float *G_dev = nullptr;   // [F][B] as given
   bf::v4i *W = nullptr;     // [F][KS][NTP][64] weight operands
@@ -45,6 +54,15 @@ extern thread_local std::string g_last_error;
 int fail(int status, const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
 int check_shape(const bf_shape *s);
-bool choose_mfma_config(const bf_shape &s, int num_cus, MfmaConfig *cfg);
+bool choose_mfma_config(const bf_shape &s, int num_cus, MfmaConfig *cfg, const TuneChoice *force);
+bool choose_tiling(int64_t ndishes, int64_t nbeams, int forced_nt, MfmaConfig *cfg);
+bool choose_grid(int64_t ntimes, int64_t nfrequencies, int64_t target_wgs, MfmaConfig *cfg);
+int wgs_per_cu(const MfmaConfig &cfg);
+void remember_tuning(const bf_shape &s, const TuneChoice &c);
+bool recall_tuning(const bf_shape &s, TuneChoice *c);
+int alloc_weight_tables(bf_plan *p);
+int prepare_weights(bf_plan *p, hipStream_t st);
+int launch_mfma(const bf_plan *p, const MfmaConfig &c, uint8_t *J, const uint8_t *E, int64_t T, int F, int wrow0,
+                hipStream_t st);
 int launch_form_beams(const bf_plan *p, uint8_t *J, const uint8_t *E, hipStream_t st);
 } // namespace bf

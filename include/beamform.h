@@ -128,6 +128,70 @@ uint32_t bf_adler32(const uint8_t *data, size_t len);
  * for `seed`. Used to build large synthetic inputs in place. */
 int bf_fill_uniform_device(uint8_t *buf, size_t n, uint64_t seed, uint64_t offset, void *stream);
 
+/* ---------------------------------------------------------------------------
+ * Channel tables, batching and tuning. These have no counterpart in the
+ * reference, which runs one compile-time chunk per call (cpu.cxx:48-67,
+ * cuda4.cu:1568-1580 launches one kernel per stream); they serve the
+ * telescope's uneven per-channel chunks.
+ *
+ * A "channel-table plan" is an ordinary plan whose frequencies are the
+ * channels: A[c][B][D], G[c][B]; its ntimes is not used by the calls below.
+ * ------------------------------------------------------------------------- */
+
+/* form_beams for ONE channel of a plan, with its own ntimes: E is
+ * [ntimes][1][D][P], J is [B][1][P][ntimes] (or [B][1][ntimes][P]). This is
+ * the plain per-chunk launch. Asynchronous on `stream`. */
+int bf_form_beams_channel(const bf_plan *plan, int channel, int64_t ntimes, uint8_t *J, const uint8_t *E,
+                          void *stream);
+
+typedef struct bf_chunk {
+  const uint8_t *E; /* device: [ntimes][1][D][P] */
+  uint8_t *J;       /* device: [B][1][P][ntimes] or [B][1][ntimes][P] per the plan layout */
+  int64_t ntimes;   /* >= 1 */
+  int32_t channel;  /* row of the plan's channel table, 0 <= channel < F */
+  int32_t reserved; /* must be 0 */
+} bf_chunk;
+
+typedef struct bf_batch bf_batch;
+
+/* Creates a batch context for up to max_chunks chunks of a channel-table
+ * plan: device storage for the chunk descriptors and the work queue, and a
+ * hipGraph of the launch, captured on first use of bf_batch_run_graph. */
+int bf_batch_create(const bf_plan *channel_plan, int max_chunks, bf_batch **batch);
+int bf_batch_destroy(bf_batch *batch);
+
+/* Work-item size in 16-sample tiles (rounded to a multiple of 8; 0 restores
+ * the default). Smaller items balance better, larger ones restage weights
+ * less often. */
+int bf_batch_set_item_tiles(bf_batch *batch, int item_tiles);
+
+/* Runs all chunks in one persistent launch with a device-side work queue, on
+ * `stream` (asynchronous; the chunk list is copied before the call returns).
+ * The plan must use the MFMA path (P = 2, D a multiple of 8 up to 1024, gains
+ * in the fast range); otherwise BF_ERR_UNSUPPORTED. */
+int bf_batch_run(bf_batch *batch, const bf_chunk *chunks, int nchunks, void *stream);
+
+/* Same work, launched as a replay of the batch's captured hipGraph (queue
+ * reset + persistent kernel); only the chunk list and work items are copied
+ * per call. */
+int bf_batch_run_graph(bf_batch *batch, const bf_chunk *chunks, int nchunks, void *stream);
+
+/* One-line description of the batch configuration. */
+int bf_batch_describe(const bf_batch *batch, char *buf, size_t len);
+
+/* Times candidate kernel configurations (beam tiles per workgroup, grid size)
+ * on the caller's device buffers E and J with the plan's shape, keeps the
+ * fastest in the plan, and remembers it for later plans of the same shape in
+ * this process. `iters` timed launches per candidate (0 = default). A report
+ * of the candidates is written to report (may be NULL). Synchronous. */
+int bf_plan_autotune(bf_plan *plan, uint8_t *J, const uint8_t *E, void *stream, int iters, char *report,
+                     size_t report_len);
+
+/* Times candidate work-item sizes for the given chunk list (one persistent
+ * launch each, `iters` times) and keeps the fastest. Synchronous. */
+int bf_batch_autotune(bf_batch *batch, const bf_chunk *chunks, int nchunks, void *stream, int iters, char *report,
+                      size_t report_len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,176 @@
+"""GPU parity of the channel-table paths: one-channel launches, the batched
+persistent launch (plain and hipGraph replay) and the autotuners. Every chunk
+is compared bit for bit with the oracle run on that chunk alone (F = 1 with
+the channel's weights and gains)."""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import beamforming_amd as bf
+from oracle import bf_oracle as bo
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def channel_table(C, D, B, seed, gain_scale=8.0):
+    A = bo.fill_uniform(C * B * D, seed)
+    G = bo.setup_gain(C * B, D) * np.float32(gain_scale)
+    return A, G
+
+
+def oracle_chunk(E, A, G, channel, T, D, B, layout=0):
+    Ac = A.reshape(-1, B * D)[channel]
+    Gc = G.reshape(-1, B)[channel]
+    return bo.form_beams(E, Ac, Gc, T, 1, D, B, jlayout=layout)
+
+
+def make_chunks(specs, D, B, seed):
+    """specs: list of (ntimes, channel). Returns host E arrays, device E/J tensors."""
+    host, dev = [], []
+    for i, (T, c) in enumerate(specs):
+        E = bo.fill_uniform(T * D * 2, seed + 17 * i)
+        dE = torch.from_numpy(E).cuda()
+        dJ = torch.zeros(B * 2 * T, dtype=torch.uint8, device="cuda")
+        host.append(E)
+        dev.append((dE, dJ))
+    return host, dev
+
+
+def chunk_list(specs, dev):
+    return [(dE.data_ptr(), dJ.data_ptr(), T, c) for (T, c), (dE, dJ) in zip(specs, dev)]
+
+
+def check_chunks(specs, host, dev, A, G, D, B, layout=0):
+    for i, ((T, c), E, (dE, dJ)) in enumerate(zip(specs, host, dev)):
+        got = dJ.cpu().numpy()
+        assert (got != 0).all(), f"chunk {i}: unwritten outputs"
+        want = oracle_chunk(E, A, G, c, T, D, B, layout)
+        bad = np.flatnonzero(got != want)
+        assert bad.size == 0, f"chunk {i} (T={T}, channel={c}): {bad.size} mismatches"
+
+
+@pytest.mark.parametrize("layout", [bf.BF_J_BFPT, bf.BF_J_BFTP])
+def test_channel_launch(layout):
+    C, D, B = 5, 512, 96
+    A, G = channel_table(C, D, B, 3)
+    shape = bf.Shape.make(1, C, D, B, j_layout=layout)
+    specs = [(1000, 0), (16, 4), (3, 2), (4096, 3), (777, 1)]
+    host, dev = make_chunks(specs, D, B, 100)
+    with bf.Plan(shape, A, G) as plan:
+        say(f"channel plan: {plan.describe()}")
+        for (T, c), (dE, dJ) in zip(specs, dev):
+            plan.form_beams_channel(c, T, dJ.data_ptr(), dE.data_ptr(), stream())
+        torch.cuda.synchronize()
+    check_chunks(specs, host, dev, A, G, D, B, layout)
+
+
+@pytest.mark.parametrize("D,B", [(512, 96), (64, 16), (512, 512), (40, 50)])
+def test_batch_matches_oracle(D, B):
+    C = 7
+    A, G = channel_table(C, D, B, 5)
+    shape = bf.Shape.make(1, C, D, B)
+    rng = np.random.default_rng(D + B)
+    specs = [(int(rng.integers(1, 3000)), int(rng.integers(0, C))) for _ in range(11)] + [(1, 0), (16, 6), (4097, 3)]
+    host, dev = make_chunks(specs, D, B, 200)
+    with bf.Plan(shape, A, G) as plan, bf.Batch(plan, 32) as batch:
+        say(f"batch D={D} B={B}")
+        batch.run(chunk_list(specs, dev), stream())
+        torch.cuda.synchronize()
+        say(f"  {batch.describe()}")
+    check_chunks(specs, host, dev, A, G, D, B)
+
+
+def test_batch_graph_replays_match_eager():
+    C, D, B = 8, 512, 96
+    A, G = channel_table(C, D, B, 9)
+    shape = bf.Shape.make(1, C, D, B)
+    rng = np.random.default_rng(4)
+    lists = []
+    for r in range(3):  # different chunk lists through the same captured graph
+        specs = [(int(rng.integers(100, 5000)), int(rng.integers(0, C))) for _ in range(6 + 3 * r)]
+        lists.append(specs)
+    with bf.Plan(shape, A, G) as plan, bf.Batch(plan, 16) as batch:
+        for r, specs in enumerate(lists):
+            host, dev = make_chunks(specs, D, B, 300 + 50 * r)
+            for rep in range(3):
+                for _, dJ in dev:
+                    dJ.zero_()
+                torch.cuda.synchronize()
+                batch.run(chunk_list(specs, dev), stream(), graph=True)
+                torch.cuda.synchronize()
+                say(f"  replay {r}.{rep}: {batch.describe()}")
+                check_chunks(specs, host, dev, A, G, D, B)
+                # eager work between replays
+                batch.run(chunk_list(specs[:2], dev[:2]), stream())
+                torch.cuda.synchronize()
+
+
+def test_batch_item_sizes_and_autotune():
+    C, D, B = 4, 512, 96
+    A, G = channel_table(C, D, B, 11)
+    shape = bf.Shape.make(1, C, D, B)
+    specs = [(2048, 0), (512, 1), (8192, 2), (100, 3), (3000, 1)]
+    host, dev = make_chunks(specs, D, B, 400)
+    with bf.Plan(shape, A, G) as plan, bf.Batch(plan, 8) as batch:
+        for tiles in (8, 24, 64, 256, 0):
+            batch.set_item_tiles(tiles)
+            for _, dJ in dev:
+                dJ.zero_()
+            batch.run(chunk_list(specs, dev), stream())
+            torch.cuda.synchronize()
+            check_chunks(specs, host, dev, A, G, D, B)
+        report = batch.autotune(chunk_list(specs, dev), stream(), iters=2)
+        say(report)
+        assert "chosen" in report
+        for _, dJ in dev:
+            dJ.zero_()
+        batch.run(chunk_list(specs, dev), stream())
+        torch.cuda.synchronize()
+    check_chunks(specs, host, dev, A, G, D, B)
+
+
+def test_batch_rejects_bad_chunks():
+    C, D, B = 2, 64, 16
+    A, G = channel_table(C, D, B, 13)
+    shape = bf.Shape.make(1, C, D, B)
+    dE = torch.zeros(64 * D * 2 + 16, dtype=torch.uint8, device="cuda")
+    dJ = torch.zeros(B * 2 * 64 + 16, dtype=torch.uint8, device="cuda")
+    with bf.Plan(shape, A, G) as plan, bf.Batch(plan, 2) as batch:
+        with pytest.raises(bf.BeamformError):
+            batch.run([(dE.data_ptr() + 1, dJ.data_ptr(), 64, 0)], stream())  # misaligned E
+        with pytest.raises(bf.BeamformError):
+            batch.run([(dE.data_ptr(), dJ.data_ptr(), 64, 5)], stream())  # channel out of range
+        with pytest.raises(bf.BeamformError):
+            batch.run([(dE.data_ptr(), dJ.data_ptr(), 64, 0)] * 3, stream())  # more than max_chunks
+        batch.run([], stream())  # empty list is a no-op
+        torch.cuda.synchronize()
+
+
+def test_plan_autotune_keeps_results_exact():
+    T, F, D, B = 4096, 4, 512, 96
+    shape = bf.Shape.make(T, F, D, B)
+    E = bo.fill_uniform(T * F * D * 2, 21)
+    A = bo.fill_uniform(F * B * D, 22)
+    G = bo.setup_gain(F * B, D) * np.float32(8)
+    dE = torch.from_numpy(E).cuda()
+    dJ = torch.zeros(shape.j_bytes, dtype=torch.uint8, device="cuda")
+    with bf.Plan(shape, A, G) as plan:
+        report = plan.autotune(dJ.data_ptr(), dE.data_ptr(), stream(), iters=2)
+        say(report)
+        assert "chosen" in report
+        dJ.zero_()
+        plan.form_beams_device(dJ.data_ptr(), dE.data_ptr(), stream())
+        torch.cuda.synchronize()
+        say(plan.describe())
+    assert np.array_equal(dJ.cpu().numpy(), bo.form_beams(E, A, G, T, F, D, B))
