@@ -12,7 +12,7 @@ namespace bf {
 struct MfmaConfig {
   int ks = 0;              // K steps (power of two, 16 dishes each)
   bool dish_tail = false;  // ks * 16 != ndishes: masked last steps
-  int nt = 0;              // 16-beam tiles per workgroup
+  int nt = 0;              // 32-beam tiles per workgroup
   int nbg = 0;             // beam groups
   int ntp = 0;             // nt * nbg
   int tiles_per_chunk = 0; // 16-sample tiles per workgroup
