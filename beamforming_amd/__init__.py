@@ -19,7 +19,15 @@ import numpy as np
 __all__ = [
     "BF_J_BFPT", "BF_J_BFTP", "BeamformError", "Shape", "Plan", "lib", "library_path",
     "form_beams", "adler32", "adler32_device", "fill_uniform_device", "version", "Chunk", "Batch",
+    "Server", "ServerConfig", "Request", "Result", "ServerCounters",
+    "BF_ERR_INVALID_ARGUMENT", "BF_ERR_UNSUPPORTED", "BF_ERR_OVER_BUDGET", "BF_ERR_NOT_READY", "BF_ERR_CLOSED",
 ]
+
+BF_ERR_INVALID_ARGUMENT = 1
+BF_ERR_UNSUPPORTED = 2
+BF_ERR_OVER_BUDGET = 5
+BF_ERR_NOT_READY = 6
+BF_ERR_CLOSED = 7
 
 BF_J_BFPT = 0  # J[b][f][p][t] (reference Jlinear, arraysizes.hxx:91-101)
 BF_J_BFTP = 1  # J[b][f][t][p] (reference J2linear, arraysizes.hxx:103-112)
@@ -74,15 +82,81 @@ class Shape(ctypes.Structure):
 
 
 class Chunk(ctypes.Structure):
-    """bf_chunk: one chunk of a batch (device E and J, its ntimes and channel)."""
+    """bf_chunk: one chunk of a batch (device E and J, its ntimes, first
+    channel and number of consecutive channels)."""
 
     _fields_ = [
         ("E", ctypes.c_void_p),
         ("J", ctypes.c_void_p),
         ("ntimes", ctypes.c_int64),
         ("channel", ctypes.c_int32),
+        ("nfreq", ctypes.c_int32),
+    ]
+
+
+class ServerConfig(ctypes.Structure):
+    """bf_server_config."""
+
+    _fields_ = [
+        ("hbm_budget", ctypes.c_size_t),
+        ("max_inflight", ctypes.c_int32),
+        ("max_batch", ctypes.c_int32),
+        ("max_launches", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
     ]
+
+
+class Request(ctypes.Structure):
+    """bf_request: one chunk for the server."""
+
+    _fields_ = [
+        ("E", ctypes.c_void_p),
+        ("ntimes", ctypes.c_int64),
+        ("channel", ctypes.c_int32),
+        ("nfreq", ctypes.c_int32),
+        ("e_on_host", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("tag", ctypes.c_uint64),
+    ]
+
+
+class Result(ctypes.Structure):
+    """bf_result: a finished request."""
+
+    _fields_ = [
+        ("ticket", ctypes.c_uint64),
+        ("tag", ctypes.c_uint64),
+        ("J", ctypes.c_void_p),
+        ("j_bytes", ctypes.c_size_t),
+        ("status", ctypes.c_int32),
+        ("batch_size", ctypes.c_int32),
+        ("t_submit", ctypes.c_double),
+        ("t_admit", ctypes.c_double),
+        ("t_launch", ctypes.c_double),
+        ("t_done", ctypes.c_double),
+    ]
+
+    @property
+    def latency(self):
+        return self.t_done - self.t_submit
+
+
+class ServerCounters(ctypes.Structure):
+    """bf_server_counters."""
+
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "submitted", "admitted", "completed", "released", "rejected", "failed",
+        "launches", "max_batch", "waiting", "inflight")] + [
+        ("arena_bytes", ctypes.c_size_t),
+        ("bytes_in_use", ctypes.c_size_t),
+        ("peak_bytes", ctypes.c_size_t),
+        ("mean_batch", ctypes.c_double),
+        ("clock", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 def _load():
@@ -116,6 +190,17 @@ def _load():
         "bf_plan_autotune": (ctypes.c_int, [vp, u8p, u8p, vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]),
         "bf_batch_autotune": (ctypes.c_int, [vp, ctypes.POINTER(Chunk), ctypes.c_int, vp, ctypes.c_int,
                                              ctypes.c_char_p, ctypes.c_size_t]),
+        "bf_server_create": (ctypes.c_int, [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
+        "bf_server_destroy": (ctypes.c_int, [vp]),
+        "bf_server_submit": (ctypes.c_int, [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
+        "bf_server_submit_many": (ctypes.c_int, [vp, ctypes.POINTER(Request), ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_uint64)]),
+        "bf_server_poll": (ctypes.c_int, [vp, ctypes.POINTER(Result)]),
+        "bf_server_wait": (ctypes.c_int, [vp, ctypes.c_double, ctypes.POINTER(Result)]),
+        "bf_server_read": (ctypes.c_int, [vp, ctypes.c_uint64, u8p, ctypes.c_size_t]),
+        "bf_server_release": (ctypes.c_int, [vp, ctypes.c_uint64]),
+        "bf_server_stats": (ctypes.c_int, [vp, ctypes.POINTER(ServerCounters)]),
+        "bf_server_clock": (ctypes.c_double, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -250,14 +335,14 @@ def _chunk_array(chunks):
         if isinstance(c, Chunk):
             arr[i] = c
         else:
-            E, J, ntimes, channel = c
-            arr[i] = Chunk(E, J, ntimes, channel, 0)
+            E, J, ntimes, channel = c[:4]
+            arr[i] = Chunk(E, J, ntimes, channel, c[4] if len(c) > 4 else 1)
     return arr
 
 
 class Batch:
-    """A batch context over a channel-table plan: many chunks (each one
-    channel, own ntimes) per persistent launch with a device-side queue."""
+    """A batch context over a channel-table plan: many chunks (each with its
+    own ntimes and channels) per persistent launch with a device-side queue."""
 
     def __init__(self, plan, max_chunks):
         self.plan = plan  # keeps the plan alive
@@ -285,7 +370,7 @@ class Batch:
         _check(lib.bf_batch_set_item_tiles(self._h, n))
 
     def run(self, chunks, stream=0, graph=False):
-        """chunks: list of Chunk or (E_ptr, J_ptr, ntimes, channel)."""
+        """chunks: list of Chunk or (E_ptr, J_ptr, ntimes, channel[, nfreq])."""
         arr = _chunk_array(chunks)
         fn = lib.bf_batch_run_graph if graph else lib.bf_batch_run
         _check(fn(self._h, arr, len(chunks), ctypes.c_void_p(stream)))
@@ -300,3 +385,84 @@ class Batch:
         buf = ctypes.create_string_buffer(512)
         _check(lib.bf_batch_describe(self._h, buf, 512))
         return buf.value.decode()
+
+
+class Server:
+    """The streaming request loop over a channel-table plan (continuous
+    batching with HBM-budget admission)."""
+
+    def __init__(self, plan, hbm_budget, max_inflight=0, max_batch=0, max_launches=0, use_graph=False):
+        self.plan = plan  # the plan must outlive the server
+        cfg = ServerConfig(hbm_budget, max_inflight, max_batch, max_launches, int(bool(use_graph)), 0)
+        self._h = ctypes.c_void_p()
+        _check(lib.bf_server_create(plan.handle, ctypes.byref(cfg), ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib.bf_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, E_ptr, ntimes, channel, nfreq=1, tag=0, e_on_host=False):
+        """Queues one chunk; returns its ticket."""
+        req = Request(E_ptr, ntimes, channel, nfreq, int(bool(e_on_host)), 0, tag)
+        t = ctypes.c_uint64()
+        _check(lib.bf_server_submit(self._h, ctypes.byref(req), ctypes.byref(t)))
+        return t.value
+
+    def submit_many(self, requests):
+        """Queues several chunks atomically: requests is a list of tuples
+        (E_ptr, ntimes, channel, nfreq, tag[, e_on_host]). Returns tickets."""
+        n = len(requests)
+        arr = (Request * max(1, n))()
+        for i, q in enumerate(requests):
+            arr[i] = Request(q[0], q[1], q[2], q[3], int(bool(q[5])) if len(q) > 5 else 0, 0, q[4])
+        tickets = (ctypes.c_uint64 * max(1, n))()
+        _check(lib.bf_server_submit_many(self._h, arr, n, tickets))
+        return list(tickets[:n])
+
+    def poll(self):
+        """A finished request (Result) or None."""
+        r = Result()
+        st = lib.bf_server_poll(self._h, ctypes.byref(r))
+        if st == BF_ERR_NOT_READY:
+            return None
+        _check(st)
+        return r
+
+    def wait(self, timeout=-1.0):
+        """Waits for a finished request; None on timeout."""
+        r = Result()
+        st = lib.bf_server_wait(self._h, float(timeout), ctypes.byref(r))
+        if st == BF_ERR_NOT_READY:
+            return None
+        _check(st)
+        return r
+
+    def read(self, ticket, nbytes):
+        """Copies a finished request's J to a new host array."""
+        out = np.empty(nbytes, dtype=np.uint8)
+        _check(lib.bf_server_read(self._h, ticket, out.ctypes.data_as(ctypes.c_void_p), nbytes))
+        return out
+
+    def release(self, ticket):
+        _check(lib.bf_server_release(self._h, ticket))
+
+    def stats(self):
+        c = ServerCounters()
+        _check(lib.bf_server_stats(self._h, ctypes.byref(c)))
+        return c.as_dict()
+
+    def clock(self):
+        return lib.bf_server_clock(self._h)

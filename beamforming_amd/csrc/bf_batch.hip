@@ -48,6 +48,8 @@ int check_channel_args(const bf_plan *p, int channel, int64_t ntimes, const void
   return BF_OK;
 }
 
+int chunk_nfreq(const bf_chunk &k) { return k.nfreq == 0 ? 1 : k.nfreq; }
+
 void free_graph(bf_batch *b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -92,11 +94,14 @@ int upload(bf_batch *b, const bf_chunk *chunks, int n, hipStream_t st) {
   int64_t total_tiles = 0;
   for (int i = 0; i < n; ++i) {
     const bf_chunk &k = chunks[i];
-    if (!k.E || !k.J || k.ntimes < 1 || k.reserved != 0 || k.channel < 0 || k.channel >= p->shape.nfrequencies)
-      return fail(BF_ERR_INVALID_ARGUMENT, "bad chunk " + std::to_string(i));
+    const int64_t nf = chunk_nfreq(k);
+    if (!k.E || !k.J || k.ntimes < 1 || nf < 1 || nf > 65535 || k.channel < 0 ||
+        k.channel + nf > p->shape.nfrequencies)
+      return fail(BF_ERR_INVALID_ARGUMENT, "bad chunk " + std::to_string(i) +
+                                               " (NULL buffer, ntimes < 1, or channels outside the table)");
     if (((uintptr_t)k.E & 15u) || ((uintptr_t)k.J & 7u))
       return fail(BF_ERR_INVALID_ARGUMENT, "chunk " + std::to_string(i) + ": E must be 16-byte and J 8-byte aligned");
-    total_tiles += (k.ntimes + kTileT - 1) / kTileT * c.nbg;
+    total_tiles += (k.ntimes + kTileT - 1) / kTileT * c.nbg * nf;
   }
   const int it = b->item_tiles > 0 ? b->item_tiles : default_item_tiles(total_tiles, b->grid);
   std::vector<BatchItem> items;
@@ -104,8 +109,11 @@ int upload(bf_batch *b, const bf_chunk *chunks, int n, hipStream_t st) {
   for (int i = 0; i < n; ++i) {
     const int64_t nt = (chunks[i].ntimes + kTileT - 1) / kTileT;
     if (nt > (1ll << 30)) return fail(BF_ERR_UNSUPPORTED, "chunk too long");
-    for (int bg = 0; bg < c.nbg; ++bg)
-      for (int64_t t = 0; t < nt; t += it) items.push_back(BatchItem{i, bg, (int32_t)t, (int32_t)std::min(nt, t + it)});
+    const int nf = chunk_nfreq(chunks[i]);
+    for (int f = 0; f < nf; ++f)
+      for (int bg = 0; bg < c.nbg; ++bg)
+        for (int64_t t = 0; t < nt; t += it)
+          items.push_back(BatchItem{i, (uint16_t)f, (uint16_t)bg, (int32_t)t, (int32_t)std::min(nt, t + it)});
   }
   std::stable_sort(items.begin(), items.end(), [](const BatchItem &x, const BatchItem &y) {
     return x.tile_end - x.tile_begin > y.tile_end - y.tile_begin;
@@ -116,7 +124,7 @@ int upload(bf_batch *b, const bf_chunk *chunks, int n, hipStream_t st) {
     b->upload_pending = false;
   }
   for (int i = 0; i < n; ++i)
-    b->h_chunks[i] = BatchChunk{chunks[i].E, chunks[i].J, chunks[i].ntimes, chunks[i].channel, 0};
+    b->h_chunks[i] = BatchChunk{chunks[i].E, chunks[i].J, chunks[i].ntimes, chunks[i].channel, chunk_nfreq(chunks[i])};
   std::copy(items.begin(), items.end(), b->h_items);
   *b->h_nitems = (int)items.size();
   if (n > 0) BF_HIP(hipMemcpyAsync(b->d_chunks, b->h_chunks, (size_t)n * sizeof(BatchChunk), hipMemcpyHostToDevice, st));
@@ -210,6 +218,7 @@ int bf_batch_create(const bf_plan *p, int max_chunks, bf_batch **out) {
   if (!p || max_chunks < 1) return fail(BF_ERR_INVALID_ARGUMENT, "NULL plan or max_chunks < 1");
   if (!p->use_mfma)
     return fail(BF_ERR_UNSUPPORTED, "batching needs the MFMA path (P = 2, D % 8 == 0, D <= 1024, gains in range)");
+  if (p->cfg.nbg > 65535) return fail(BF_ERR_UNSUPPORTED, "too many beam groups for a batch work item");
   bf_batch *b = new bf_batch();
   b->plan = p;
   b->max_chunks = max_chunks;

@@ -330,20 +330,23 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
 }
 
 // ---------------------------------------------------------------------------
-// Batched form_beams: independent chunks (one channel each, own T) in one
-// persistent launch. The host builds a list of work items (chunk, beam group,
-// tile range), longest first; workgroups take items from a device-side queue
-// (an atomic head counter) until it is empty, restaging the weights in LDS
-// only when the (channel, beam group) changes.
+// Batched form_beams: independent chunks (nf consecutive channels each, own
+// T) in one persistent launch. The host builds a list of work items (chunk,
+// frequency of the chunk, beam group, tile range), longest first; workgroups
+// take items from a device-side queue (an atomic head counter) until it is
+// empty, restaging the weights in LDS only when the (channel, beam group)
+// changes.
 struct BatchChunk {
-  const uint8_t *E; // [T][1][D][2]
-  uint8_t *J;       // [B][1][2][T] or [B][1][T][2]
+  const uint8_t *E; // [T][nf][D][2]
+  uint8_t *J;       // [B][nf][2][T] or [B][nf][T][2]
   int64_t T;
-  int32_t channel;
-  int32_t pad;
+  int32_t channel;  // channel-table row of frequency 0 of the chunk
+  int32_t nf;       // frequencies in the chunk
 };
 struct BatchItem {
-  int32_t chunk, bg, tile_begin, tile_end;
+  int32_t chunk;
+  uint16_t f, bg; // frequency within the chunk, beam group
+  int32_t tile_begin, tile_end;
 };
 struct BatchParams {
   const BatchChunk *chunks;
@@ -373,19 +376,20 @@ __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a
     if (it >= nitems) break; // uniform: every thread read the same s_item
     const BatchItem item = a.items[it];
     const BatchChunk ch = a.chunks[item.chunk];
-    if (ch.channel != loaded_channel || item.bg != loaded_bg) {
-      stage_weights<NT, KS>(a.W, a.corr, a.G, a.NTP, ch.channel, item.bg, Wl, Cl, Gl);
-      loaded_channel = ch.channel;
+    const int channel = ch.channel + item.f;
+    if (channel != loaded_channel || item.bg != loaded_bg) {
+      stage_weights<NT, KS>(a.W, a.corr, a.G, a.NTP, channel, item.bg, Wl, Cl, Gl);
+      loaded_channel = channel;
       loaded_bg = item.bg;
     }
     __syncthreads();
     TileJob job;
-    job.E = ch.E;
-    job.e_tstride = (int64_t)a.D * 2;
+    job.E = ch.E + (int64_t)item.f * a.D * 2;
+    job.e_tstride = (int64_t)ch.nf * a.D * 2;
     job.J = ch.J;
     job.T = ch.T;
-    job.F = 1;
-    job.f = 0;
+    job.F = ch.nf;
+    job.f = item.f;
     job.D = a.D;
     job.B = a.B;
     job.bg = item.bg;

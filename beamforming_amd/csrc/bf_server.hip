@@ -1,0 +1,438 @@
+// bf_server.hip -- the streaming request loop (continuous batching).
+//
+// Requests (one chunk each: E of nfreq consecutive channels, own ntimes) are
+// submitted from any thread. A loop thread owned by the server
+//   1. admits waiting requests in arrival order while a slot is free and the
+//      HBM arena (the budget, one device allocation made at creation) has
+//      room for the request's J and, for host-fed E, its staging copy;
+//   2. launches everything admitted so far as ONE persistent batched launch
+//      (work queue over all chunks, longest items first) on a free launch
+//      stream, up to max_launches launches in flight; a launch issued while
+//      another is still draining starts on the CUs that the other's
+//      workgroups leave, so new requests overlap the tail of earlier ones;
+//   3. polls the launches' events, stamps the finished requests and hands
+//      them to bf_server_poll / bf_server_wait.
+// A result's J stays in the arena until bf_server_release; memory returned
+// there (or by finished E staging) lets the next waiting requests in.
+#include "beamform.h"
+#include "bf_plan.hpp"
+#include "bf_sched.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+using namespace bf;
+
+namespace {
+
+struct Request {
+  uint64_t ticket = 0, tag = 0;
+  const uint8_t *E_src = nullptr;
+  bool e_on_host = false;
+  int64_t ntimes = 0;
+  int channel = 0, nfreq = 1;
+  size_t e_bytes = 0, j_bytes = 0;
+  size_t j_off = 0, e_off = 0;
+  bool has_e_piece = false, has_j_piece = false;
+  double t_submit = 0, t_admit = 0, t_launch = 0, t_done = 0;
+  int status = BF_OK;
+  int batch_size = 0;
+  bool done = false;
+};
+
+struct Launch {
+  bf_batch *batch = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  bool busy = false;
+  std::vector<uint64_t> tickets;
+};
+
+} // namespace
+
+struct bf_server {
+  const bf_plan *plan = nullptr;
+  bf_server_config cfg{};
+  int device = 0;
+  uint8_t *arena = nullptr;
+  ArenaAllocator alloc;
+  AdmissionQueue<uint64_t> *adm = nullptr;
+  std::vector<Launch> launches;
+  std::chrono::steady_clock::time_point epoch;
+
+  std::mutex mu;
+  std::condition_variable cv_work; // new requests, releases, shutdown
+  std::condition_variable cv_done; // finished requests
+  std::unordered_map<uint64_t, Request> reqs;
+  std::deque<uint64_t> ready;     // admitted, not launched
+  std::deque<uint64_t> finished;  // done, not yet returned
+  uint64_t next_ticket = 1;
+  bool stopping = false;
+  std::thread loop;
+  std::string loop_error;
+
+  uint64_t n_submitted = 0, n_admitted = 0, n_completed = 0, n_released = 0, n_rejected = 0, n_launches = 0;
+  uint64_t n_failed = 0, max_batch_seen = 0, batch_requests = 0;
+
+  double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - epoch).count(); }
+};
+
+namespace {
+
+// Issues one launch: staging copies of host-fed E, then the batched kernel,
+// then the completion event. Called without the server lock.
+int issue(bf_server *s, Launch &L, const std::vector<bf_chunk> &chunks,
+          const std::vector<std::pair<uint8_t *, const Request *>> &copies) {
+  for (const auto &c : copies) {
+    const hipError_t e =
+        hipMemcpyAsync(c.first, c.second->E_src, c.second->e_bytes, hipMemcpyHostToDevice, L.stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync (E staging)");
+  }
+  const int rc = s->cfg.use_graph ? bf_batch_run_graph(L.batch, chunks.data(), (int)chunks.size(), L.stream)
+                                  : bf_batch_run(L.batch, chunks.data(), (int)chunks.size(), L.stream);
+  if (rc) return rc;
+  const hipError_t e = hipEventRecord(L.done, L.stream);
+  if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+  return BF_OK;
+}
+
+// Marks the requests of a launch finished (status rc); lock held.
+void finish(bf_server *s, Launch &L, int rc, double t) {
+  for (uint64_t id : L.tickets) {
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) continue;
+    Request &r = it->second;
+    r.t_done = t;
+    r.status = rc;
+    r.done = true;
+    if (r.has_e_piece) { // the staged E is no longer needed
+      s->alloc.release(r.e_off);
+      r.has_e_piece = false;
+    }
+    s->adm->finish_slot();
+    s->finished.push_back(id);
+    ++s->n_completed;
+    if (rc) ++s->n_failed;
+  }
+  L.tickets.clear();
+  L.busy = false;
+}
+
+void admit_locked(bf_server *s) {
+  s->adm->admit([&](uint64_t id, const AdmitGrant &g) {
+    Request &r = s->reqs[id];
+    r.j_off = g.offset[0];
+    r.has_j_piece = true;
+    if (r.e_on_host) {
+      r.e_off = g.offset[1];
+      r.has_e_piece = true;
+    }
+    r.t_admit = s->now();
+    s->ready.push_back(id);
+    ++s->n_admitted;
+  });
+}
+
+void run_loop(bf_server *s) {
+  if (hipSetDevice(s->device) != hipSuccess) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->loop_error = "hipSetDevice failed in the server thread";
+  }
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    bool progressed = false;
+    if (!s->stopping) admit_locked(s); // shutting down: only finish what is on the device
+
+    // Launch what is admitted, on a free launch slot.
+    int slot = -1;
+    for (size_t i = 0; i < s->launches.size(); ++i)
+      if (!s->launches[i].busy) {
+        slot = (int)i;
+        break;
+      }
+    if (slot >= 0 && !s->ready.empty() && !s->stopping) {
+      Launch &L = s->launches[slot];
+      std::vector<bf_chunk> chunks;
+      std::vector<std::pair<uint8_t *, const Request *>> copies;
+      const double t = s->now();
+      while (!s->ready.empty() && (int)chunks.size() < s->cfg.max_batch) {
+        const uint64_t id = s->ready.front();
+        s->ready.pop_front();
+        Request &r = s->reqs[id];
+        r.t_launch = t;
+        const uint8_t *E = r.E_src;
+        if (r.e_on_host) {
+          E = s->arena + r.e_off;
+          copies.emplace_back(s->arena + r.e_off, &r);
+        }
+        chunks.push_back(bf_chunk{E, s->arena + r.j_off, r.ntimes, r.channel, r.nfreq});
+        L.tickets.push_back(id);
+      }
+      for (uint64_t id : L.tickets) s->reqs[id].batch_size = (int)L.tickets.size();
+      L.busy = true;
+      ++s->n_launches;
+      s->batch_requests += L.tickets.size();
+      s->max_batch_seen = std::max<uint64_t>(s->max_batch_seen, L.tickets.size());
+      lk.unlock();
+      const int rc = issue(s, L, chunks, copies); // Request pointers stay valid: entries are only erased on release
+      lk.lock();
+      if (rc) {
+        s->loop_error = g_last_error;
+        finish(s, L, rc, s->now());
+        s->cv_done.notify_all();
+      }
+      progressed = true;
+    }
+
+    // Completion.
+    bool any_busy = false;
+    for (Launch &L : s->launches) {
+      if (!L.busy) continue;
+      const hipError_t q = hipEventQuery(L.done);
+      if (q == hipErrorNotReady) {
+        any_busy = true;
+        continue;
+      }
+      const int rc = q == hipSuccess ? BF_OK : hip_fail(q, "launch");
+      if (rc) s->loop_error = g_last_error;
+      finish(s, L, rc, s->now());
+      s->cv_done.notify_all();
+      progressed = true;
+    }
+
+    if (s->stopping && !any_busy) break;
+    if (progressed) continue;
+    if (any_busy)
+      s->cv_work.wait_for(lk, std::chrono::microseconds(20));
+    else
+      s->cv_work.wait(lk);
+  }
+}
+
+void fill_result(const bf_server *s, const Request &r, bf_result *out) {
+  std::memset(out, 0, sizeof *out);
+  out->ticket = r.ticket;
+  out->tag = r.tag;
+  out->J = s->arena + r.j_off;
+  out->j_bytes = r.j_bytes;
+  out->status = r.status;
+  out->batch_size = r.batch_size;
+  out->t_submit = r.t_submit;
+  out->t_admit = r.t_admit;
+  out->t_launch = r.t_launch;
+  out->t_done = r.t_done;
+}
+
+int pop_result(bf_server *s, bf_result *out) {
+  const uint64_t id = s->finished.front();
+  s->finished.pop_front();
+  fill_result(s, s->reqs[id], out);
+  return BF_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int bf_server_create(const bf_plan *p, const bf_server_config *cfg, bf_server **out) {
+  if (!out) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  if (!p || !cfg) return fail(BF_ERR_INVALID_ARGUMENT, "NULL plan or config");
+  if (cfg->hbm_budget == 0) return fail(BF_ERR_INVALID_ARGUMENT, "hbm_budget must be > 0");
+  if (cfg->max_inflight < 0 || cfg->max_batch < 0 || cfg->max_launches < 0 || cfg->reserved != 0)
+    return fail(BF_ERR_INVALID_ARGUMENT, "negative limit or reserved field not 0");
+  bf_server *s = new bf_server();
+  s->plan = p;
+  s->cfg = *cfg;
+  if (!s->cfg.max_inflight) s->cfg.max_inflight = 256;
+  if (!s->cfg.max_batch) s->cfg.max_batch = 128;
+  if (!s->cfg.max_launches) s->cfg.max_launches = 2;
+  s->device = p->device;
+  s->epoch = std::chrono::steady_clock::now();
+  auto bail = [&](int rc) {
+    bf_server_destroy(s);
+    return rc;
+  };
+  hipError_t e;
+  if ((e = hipSetDevice(s->device)) != hipSuccess) return bail(hip_fail(e, "hipSetDevice"));
+  if ((e = hipMalloc(&s->arena, cfg->hbm_budget)) != hipSuccess) return bail(hip_fail(e, "hipMalloc (HBM budget)"));
+  s->alloc.reset(cfg->hbm_budget, 256);
+  s->adm = new AdmissionQueue<uint64_t>(&s->alloc, s->cfg.max_inflight);
+  s->launches.resize(s->cfg.max_launches);
+  for (Launch &L : s->launches) {
+    if (int rc = bf_batch_create(p, s->cfg.max_batch, &L.batch)) return bail(rc);
+    if ((e = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking)) != hipSuccess)
+      return bail(hip_fail(e, "hipStreamCreate"));
+    if ((e = hipEventCreateWithFlags(&L.done, hipEventDisableTiming)) != hipSuccess)
+      return bail(hip_fail(e, "hipEventCreate"));
+  }
+  s->loop = std::thread(run_loop, s);
+  *out = s;
+  return BF_OK;
+}
+
+int bf_server_destroy(bf_server *s) {
+  if (!s) return BF_OK;
+  if (s->loop.joinable()) {
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stopping = true;
+    }
+    s->cv_work.notify_all();
+    s->loop.join(); // returns once no launch is in flight
+  }
+  for (Launch &L : s->launches) {
+    if (L.stream) (void)hipStreamSynchronize(L.stream);
+    if (L.batch) bf_batch_destroy(L.batch);
+    if (L.done) (void)hipEventDestroy(L.done);
+    if (L.stream) (void)hipStreamDestroy(L.stream);
+  }
+  delete s->adm;
+  if (s->arena) (void)hipFree(s->arena);
+  delete s;
+  return BF_OK;
+}
+
+int bf_server_submit_many(bf_server *s, const bf_request *reqs, int n, uint64_t *tickets) {
+  if (!s || n < 0 || (n > 0 && !reqs)) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument or n < 0");
+  const bf_shape &sh = s->plan->shape;
+  std::vector<Request> rs((size_t)n);
+  std::vector<AdmitNeed> needs((size_t)n);
+  std::vector<size_t> totals((size_t)n);
+  for (int i = 0; i < n; ++i) { // validate everything before queueing anything
+    const bf_request *req = &reqs[i];
+    const int nf = req->nfreq == 0 ? 1 : req->nfreq;
+    if (!req->E || req->ntimes < 1 || nf < 1 || nf > 65535 || req->channel < 0 ||
+        (int64_t)req->channel + nf > sh.nfrequencies || req->reserved != 0 ||
+        (req->e_on_host != 0 && req->e_on_host != 1))
+      return fail(BF_ERR_INVALID_ARGUMENT, "bad request " + std::to_string(i) +
+                                               " (NULL E, ntimes < 1, channels outside the table, flags)");
+    if (!req->e_on_host && ((uintptr_t)req->E & 15u))
+      return fail(BF_ERR_INVALID_ARGUMENT, "request " + std::to_string(i) + ": device E must be 16-byte aligned");
+    Request &r = rs[i];
+    r.tag = req->tag;
+    r.E_src = req->E;
+    r.e_on_host = req->e_on_host != 0;
+    r.ntimes = req->ntimes;
+    r.channel = req->channel;
+    r.nfreq = nf;
+    r.j_bytes = (size_t)(sh.nbeams * nf * sh.npolarizations * req->ntimes);
+    r.e_bytes = (size_t)(req->ntimes * nf * sh.ndishes * sh.npolarizations);
+    needs[i].bytes[0] = r.j_bytes;
+    needs[i].bytes[1] = r.e_on_host ? r.e_bytes : 0;
+    totals[i] = s->alloc.round(needs[i].bytes[0]) + (needs[i].bytes[1] ? s->alloc.round(needs[i].bytes[1]) : 0);
+  }
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stopping) return fail(BF_ERR_CLOSED, "server is shutting down");
+    for (int i = 0; i < n; ++i)
+      if (totals[i] > s->alloc.capacity()) {
+        ++s->n_rejected;
+        return fail(BF_ERR_OVER_BUDGET, "request " + std::to_string(i) + " needs " + std::to_string(totals[i]) +
+                                            " bytes, budget is " + std::to_string(s->alloc.capacity()));
+      }
+    const double t = s->now();
+    for (int i = 0; i < n; ++i) {
+      Request &r = rs[i];
+      r.ticket = s->next_ticket++;
+      r.t_submit = t;
+      s->reqs[r.ticket] = r;
+      s->adm->push(r.ticket, needs[i]);
+      ++s->n_submitted;
+      if (tickets) tickets[i] = r.ticket;
+    }
+  }
+  s->cv_work.notify_all();
+  return BF_OK;
+}
+
+int bf_server_submit(bf_server *s, const bf_request *req, uint64_t *ticket) {
+  if (!req) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  return bf_server_submit_many(s, req, 1, ticket);
+}
+
+int bf_server_poll(bf_server *s, bf_result *out) {
+  if (!s || !out) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->finished.empty()) return BF_ERR_NOT_READY;
+  return pop_result(s, out);
+}
+
+int bf_server_wait(bf_server *s, double timeout_s, bf_result *out) {
+  if (!s || !out) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::unique_lock<std::mutex> lk(s->mu);
+  const auto pred = [&] { return !s->finished.empty(); };
+  if (timeout_s < 0)
+    s->cv_done.wait(lk, pred);
+  else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred))
+    return BF_ERR_NOT_READY;
+  return pop_result(s, out);
+}
+
+int bf_server_read(bf_server *s, uint64_t ticket, uint8_t *host, size_t len) {
+  if (!s || !host) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  uint8_t *src = nullptr;
+  size_t n = 0;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end() || !it->second.done) return fail(BF_ERR_INVALID_ARGUMENT, "unknown or unfinished ticket");
+    src = s->arena + it->second.j_off;
+    n = std::min(len, it->second.j_bytes);
+  }
+  const hipError_t e = hipMemcpy(host, src, n, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpy (result)");
+  return BF_OK;
+}
+
+int bf_server_release(bf_server *s, uint64_t ticket) {
+  if (!s) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end() || !it->second.done) return fail(BF_ERR_INVALID_ARGUMENT, "unknown or unfinished ticket");
+    Request &r = it->second;
+    if (r.has_j_piece) s->alloc.release(r.j_off);
+    if (r.has_e_piece) s->alloc.release(r.e_off);
+    // drop it from the not-yet-returned list too, if the caller never polled it
+    auto f = std::find(s->finished.begin(), s->finished.end(), ticket);
+    if (f != s->finished.end()) s->finished.erase(f);
+    s->reqs.erase(it);
+    ++s->n_released;
+  }
+  s->cv_work.notify_all();
+  return BF_OK;
+}
+
+int bf_server_stats(bf_server *s, bf_server_counters *out) {
+  if (!s || !out) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  std::memset(out, 0, sizeof *out);
+  out->submitted = s->n_submitted;
+  out->admitted = s->n_admitted;
+  out->completed = s->n_completed;
+  out->released = s->n_released;
+  out->rejected = s->n_rejected;
+  out->failed = s->n_failed;
+  out->launches = s->n_launches;
+  out->max_batch = s->max_batch_seen;
+  out->waiting = s->adm->waiting();
+  out->inflight = (uint64_t)s->adm->slots_used();
+  out->arena_bytes = s->alloc.capacity();
+  out->bytes_in_use = s->alloc.in_use();
+  out->peak_bytes = s->alloc.peak();
+  out->mean_batch = s->n_launches ? (double)s->batch_requests / (double)s->n_launches : 0.0;
+  out->clock = s->now();
+  return BF_OK;
+}
+
+double bf_server_clock(const bf_server *s) { return s ? s->now() : 0.0; }
+
+} // extern "C"

@@ -145,11 +145,11 @@ int bf_form_beams_channel(const bf_plan *plan, int channel, int64_t ntimes, uint
                           void *stream);
 
 typedef struct bf_chunk {
-  const uint8_t *E; /* device: [ntimes][1][D][P] */
-  uint8_t *J;       /* device: [B][1][P][ntimes] or [B][1][ntimes][P] per the plan layout */
+  const uint8_t *E; /* device: [ntimes][nfreq][D][P] (Elinear with F = nfreq) */
+  uint8_t *J;       /* device: [B][nfreq][P][ntimes] or [B][nfreq][ntimes][P] per the plan layout */
   int64_t ntimes;   /* >= 1 */
-  int32_t channel;  /* row of the plan's channel table, 0 <= channel < F */
-  int32_t reserved; /* must be 0 */
+  int32_t channel;  /* first row of the plan's channel table used by the chunk */
+  int32_t nfreq;    /* consecutive channels channel .. channel + nfreq - 1 (0 means 1); <= 65535 */
 } bf_chunk;
 
 typedef struct bf_batch bf_batch;
@@ -191,6 +191,96 @@ int bf_plan_autotune(bf_plan *plan, uint8_t *J, const uint8_t *E, void *stream, 
  * launch each, `iters` times) and keeps the fastest. Synchronous. */
 int bf_batch_autotune(bf_batch *batch, const bf_chunk *chunks, int nchunks, void *stream, int iters, char *report,
                       size_t report_len);
+
+/* ---------------------------------------------------------------------------
+ * Serving: a streaming request loop with continuous batching. No counterpart
+ * in the reference (its drivers run one chunk per process, cpu.cxx:48-67).
+ *
+ * Requests (one chunk of nfreq consecutive channels of a channel-table plan,
+ * with its own ntimes) are submitted from any thread. A loop thread admits
+ * them in arrival order while a slot is free and the HBM budget (one device
+ * arena allocated at creation) has room for the request's J and, for host-fed
+ * E, its staging copy; it launches everything admitted as one persistent
+ * batched launch (bf_batch_run) on one of max_launches streams, so requests
+ * arriving while others run are taken into the next launch and overlap the
+ * tail of the earlier one. J stays in the arena until bf_server_release.
+ * The plan must outlive the server.
+ * ------------------------------------------------------------------------- */
+
+typedef struct bf_server bf_server;
+
+typedef struct bf_server_config {
+  size_t hbm_budget;    /* bytes of device memory for J outputs and staged E; > 0 */
+  int32_t max_inflight; /* requests admitted and not finished at once (0 = 256) */
+  int32_t max_batch;    /* requests per launch (0 = 128) */
+  int32_t max_launches; /* launches in flight at once, one stream each (0 = 2) */
+  int32_t use_graph;    /* 1: each launch is a replay of a captured hipGraph */
+  int32_t reserved;     /* must be 0 */
+} bf_server_config;
+
+typedef struct bf_request {
+  const uint8_t *E;  /* [ntimes][nfreq][D][P]: device memory (16-byte aligned), or host if e_on_host */
+  int64_t ntimes;    /* >= 1 */
+  int32_t channel;   /* first channel-table row */
+  int32_t nfreq;     /* consecutive channels (0 means 1) */
+  int32_t e_on_host; /* 1: E is host memory, copied into the arena at launch (must stay valid until the
+                        result is returned); 0: device memory, read in place (same lifetime rule) */
+  int32_t reserved;  /* must be 0 */
+  uint64_t tag;      /* caller's value, returned with the result */
+} bf_request;
+
+typedef struct bf_result {
+  uint64_t ticket;
+  uint64_t tag;
+  uint8_t *J;         /* device: [B][nfreq][P][ntimes] (or BFTP), valid until bf_server_release */
+  size_t j_bytes;
+  int32_t status;     /* BF_OK, or the error of the launch that carried the request */
+  int32_t batch_size; /* requests in that launch */
+  double t_submit, t_admit, t_launch, t_done; /* seconds on the server clock */
+} bf_result;
+
+typedef struct bf_server_counters {
+  uint64_t submitted, admitted, completed, released, rejected, failed;
+  uint64_t launches, max_batch, waiting, inflight;
+  size_t arena_bytes, bytes_in_use, peak_bytes;
+  double mean_batch; /* requests per launch */
+  double clock;      /* server clock now, seconds */
+} bf_server_counters;
+
+/* Creates a server over a channel-table plan (MFMA path required). */
+int bf_server_create(const bf_plan *channel_plan, const bf_server_config *config, bf_server **server);
+
+/* Finishes the launches in flight, drops requests not yet launched, frees
+ * everything (J buffers of unreleased results included). NULL is accepted. */
+int bf_server_destroy(bf_server *server);
+
+/* Queues a request; returns at once. BF_ERR_OVER_BUDGET if the request alone
+ * needs more than the whole budget, BF_ERR_INVALID_ARGUMENT for a malformed
+ * request, BF_ERR_CLOSED while shutting down. */
+int bf_server_submit(bf_server *server, const bf_request *request, uint64_t *ticket);
+
+/* Queues n requests at once, atomically with respect to the loop (they are
+ * admitted and launched together as far as budget and slots allow); all or
+ * none are queued. tickets (may be NULL) receives n tickets. */
+int bf_server_submit_many(bf_server *server, const bf_request *requests, int n, uint64_t *tickets);
+
+/* Returns a finished request (each once), or BF_ERR_NOT_READY. */
+int bf_server_poll(bf_server *server, bf_result *result);
+
+/* Waits up to timeout_s seconds (< 0: without limit) for a finished request;
+ * BF_ERR_NOT_READY on timeout. */
+int bf_server_wait(bf_server *server, double timeout_s, bf_result *result);
+
+/* Copies up to len bytes of a finished request's J to host memory. */
+int bf_server_read(bf_server *server, uint64_t ticket, uint8_t *host, size_t len);
+
+/* Returns a finished request's J memory to the budget. */
+int bf_server_release(bf_server *server, uint64_t ticket);
+
+int bf_server_stats(bf_server *server, bf_server_counters *counters);
+
+/* Seconds since the server was created (the clock of bf_result). */
+double bf_server_clock(const bf_server *server);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,174 @@
+"""GPU tests of the streaming server (continuous batching, HBM-budget
+admission). Every result is compared bit for bit with the oracle run on that
+chunk alone, and a chunk's J is shown to be the same alone and in a mix."""
+import sys
+import time
+
+import numpy as np
+import pytest
+import torch
+
+import beamforming_amd as bf
+from oracle import bf_oracle as bo
+
+pytestmark = pytest.mark.gpu
+
+C, D, B = 20, 512, 96
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def table():
+    A = bo.fill_uniform(C * B * D, 31)
+    G = bo.setup_gain(C * B, D) * np.float32(8)
+    shape = bf.Shape.make(1, C, D, B)
+    plan = bf.Plan(shape, A, G)
+    yield A, G, plan
+    plan.close()
+
+
+def oracle_request(E, A, G, T, channel, nf):
+    Ac = A.reshape(C, B * D)[channel:channel + nf]
+    Gc = G.reshape(C, B)[channel:channel + nf]
+    return bo.form_beams(E, Ac, Gc, T, nf, D, B)
+
+
+class Req:
+    def __init__(self, T, channel, nf, seed):
+        self.T, self.channel, self.nf = T, channel, nf
+        self.E = bo.fill_uniform(T * nf * D * 2, seed)
+        self.dE = torch.from_numpy(self.E).cuda()
+        self.j_bytes = B * nf * 2 * T
+
+
+def random_reqs(n, seed, tmax=5000):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        nf = int(rng.choice([1, 4, 16]))
+        out.append(Req(int(rng.integers(1, tmax)), int(rng.integers(0, C - nf + 1)), nf, seed * 1000 + i))
+    return out
+
+
+def collect(server, n, timeout=30.0):
+    """Waits for n results; returns {tag: (Result, J bytes)} and releases each."""
+    got = {}
+    while len(got) < n:
+        r = server.wait(timeout)
+        assert r is not None, f"timed out with {len(got)} of {n} results; stats {server.stats()}"
+        assert r.status == 0, f"request {r.tag} failed with status {r.status}"
+        got[r.tag] = (r, server.read(r.ticket, r.j_bytes))
+        server.release(r.ticket)
+    return got
+
+
+def test_server_results_match_oracle(table):
+    A, G, plan = table
+    reqs = random_reqs(24, 1)
+    torch.cuda.synchronize()
+    with bf.Server(plan, hbm_budget=1 << 30) as srv:
+        for i, q in enumerate(reqs):
+            srv.submit(q.dE.data_ptr(), q.T, q.channel, q.nf, tag=i)
+        got = collect(srv, len(reqs))
+        st = srv.stats()
+        say(f"stats {st}")
+    assert st["completed"] == len(reqs) and st["released"] == len(reqs) and st["bytes_in_use"] == 0
+    for i, q in enumerate(reqs):
+        r, J = got[i]
+        assert r.t_submit <= r.t_admit <= r.t_launch <= r.t_done
+        want = oracle_request(q.E, A, G, q.T, q.channel, q.nf)
+        assert np.array_equal(J, want), f"request {i} (T={q.T}, channel={q.channel}, nf={q.nf})"
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_alone_equals_in_mix(table, use_graph):
+    A, G, plan = table
+    probe = Req(3001, 2, 4, 777)
+    others = random_reqs(40, 2)
+    torch.cuda.synchronize()
+    with bf.Server(plan, hbm_budget=2 << 30, use_graph=use_graph) as srv:
+        srv.submit(probe.dE.data_ptr(), probe.T, probe.channel, probe.nf, tag=1000)
+        alone = collect(srv, 1)[1000]
+        assert alone[0].batch_size == 1
+        # the mix: one atomic burst, so that the launch carries many requests
+        # at once, with the probe in the middle of it; then single requests
+        # arriving while that launch runs
+        burst = [(q.dE.data_ptr(), q.T, q.channel, q.nf, i) for i, q in enumerate(others[:30])]
+        burst.insert(15, (probe.dE.data_ptr(), probe.T, probe.channel, probe.nf, 1001))
+        srv.submit_many(burst)
+        for i, q in enumerate(others[30:]):
+            srv.submit(q.dE.data_ptr(), q.T, q.channel, q.nf, tag=30 + i)
+        mix = collect(srv, len(others) + 1)
+        st = srv.stats()
+        say(f"graph={use_graph} stats {st} probe batch {mix[1001][0].batch_size}")
+    assert mix[1001][0].batch_size > 1, "the probe was not batched with other requests"
+    assert st["max_batch"] > 1
+    assert np.array_equal(alone[1], mix[1001][1])
+    assert np.array_equal(alone[1], oracle_request(probe.E, A, G, probe.T, probe.channel, probe.nf))
+    for i, q in enumerate(others):
+        assert np.array_equal(mix[i][1], oracle_request(q.E, A, G, q.T, q.channel, q.nf)), f"request {i}"
+
+
+def test_budget_admission_and_release(table):
+    A, G, plan = table
+    reqs = [Req(4096, c, 1, 50 + c) for c in range(10)]
+    jb = reqs[0].j_bytes  # 786432 bytes each
+    budget = 2 * jb + 1000  # room for two results at a time
+    torch.cuda.synchronize()
+    with bf.Server(plan, hbm_budget=budget) as srv:
+        for i, q in enumerate(reqs):
+            srv.submit(q.dE.data_ptr(), q.T, q.channel, q.nf, tag=i)
+        time.sleep(0.2)
+        st = srv.stats()
+        say(f"before release: {st}")
+        # no result has been released, so only two can have been admitted
+        assert st["admitted"] == 2 and st["waiting"] == 8
+        got = collect(srv, len(reqs))
+        st = srv.stats()
+        say(f"after: {st}")
+        assert st["peak_bytes"] <= budget and st["bytes_in_use"] == 0
+        with pytest.raises(bf.BeamformError) as e:  # alone larger than the whole budget
+            srv.submit(reqs[0].dE.data_ptr(), 16384, 0, 1)
+        assert e.value.status == bf.BF_ERR_OVER_BUDGET
+    for i, q in enumerate(reqs):
+        assert np.array_equal(got[i][1], oracle_request(q.E, A, G, q.T, q.channel, q.nf))
+
+
+def test_host_fed_requests(table):
+    A, G, plan = table
+    reqs = random_reqs(6, 3, tmax=3000)
+    with bf.Server(plan, hbm_budget=1 << 28) as srv:
+        for i, q in enumerate(reqs):
+            srv.submit(q.E.ctypes.data, q.T, q.channel, q.nf, tag=i, e_on_host=True)
+        got = collect(srv, len(reqs))
+        assert srv.stats()["bytes_in_use"] == 0
+    for i, q in enumerate(reqs):
+        assert np.array_equal(got[i][1], oracle_request(q.E, A, G, q.T, q.channel, q.nf))
+
+
+def test_bad_requests_and_shutdown(table):
+    A, G, plan = table
+    q = Req(64, 0, 1, 9)
+    torch.cuda.synchronize()
+    with pytest.raises(bf.BeamformError):
+        bf.Server(plan, hbm_budget=0)
+    srv = bf.Server(plan, hbm_budget=1 << 26)
+    for bad in [dict(ntimes=0, channel=0, nfreq=1), dict(ntimes=64, channel=C, nfreq=1),
+                dict(ntimes=64, channel=C - 2, nfreq=4), dict(ntimes=64, channel=-1, nfreq=1)]:
+        with pytest.raises(bf.BeamformError) as e:
+            srv.submit(q.dE.data_ptr(), bad["ntimes"], bad["channel"], bad["nfreq"])
+        assert e.value.status == bf.BF_ERR_INVALID_ARGUMENT
+    with pytest.raises(bf.BeamformError):
+        srv.submit(q.dE.data_ptr() + 4, 64, 0, 1)  # misaligned device E
+    # shut down with work queued and results unreleased: must return promptly
+    big = Req(16384, 0, 4, 10)
+    torch.cuda.synchronize()
+    for i in range(8):
+        srv.submit(big.dE.data_ptr(), big.T, big.channel, big.nf, tag=i)
+    t0 = time.time()
+    srv.close()
+    assert time.time() - t0 < 10
