@@ -181,6 +181,7 @@ def _load():
         "bf_adler32": (ctypes.c_uint32, [u8p, ctypes.c_size_t]),
         "bf_fill_uniform_device": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64, vp]),
         "bf_form_beams_channel": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int64, u8p, u8p, vp]),
+        "bf_form_beams_channels": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, u8p, u8p, vp]),
         "bf_batch_create": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(vp)]),
         "bf_batch_destroy": (ctypes.c_int, [vp]),
         "bf_batch_set_item_tiles": (ctypes.c_int, [vp, ctypes.c_int]),
@@ -283,6 +284,11 @@ class Plan:
         """One channel of a channel-table plan, with its own ntimes (device pointers)."""
         _check(lib.bf_form_beams_channel(self._h, channel, ntimes, ctypes.c_void_p(J_ptr), ctypes.c_void_p(E_ptr),
                                          ctypes.c_void_p(stream)))
+
+    def form_beams_channels(self, channel, nfreq, ntimes, J_ptr, E_ptr, stream=0):
+        """nfreq consecutive channels of a channel-table plan in one launch (device pointers)."""
+        _check(lib.bf_form_beams_channels(self._h, channel, nfreq, ntimes, ctypes.c_void_p(J_ptr),
+                                          ctypes.c_void_p(E_ptr), ctypes.c_void_p(stream)))
 
     def autotune(self, J_ptr, E_ptr, stream=0, iters=0):
         """Times candidate configurations on the given device buffers, keeps the best; returns the report."""

@@ -41,9 +41,11 @@ struct bf_batch {
 
 namespace {
 
-int check_channel_args(const bf_plan *p, int channel, int64_t ntimes, const void *J, const void *E) {
+int check_channel_args(const bf_plan *p, int channel, int nfreq, int64_t ntimes, const void *J, const void *E) {
   if (!p || !J || !E) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (channel < 0 || channel >= p->shape.nfrequencies) return fail(BF_ERR_INVALID_ARGUMENT, "channel out of range");
+  if (nfreq < 1) return fail(BF_ERR_INVALID_ARGUMENT, "nfreq must be >= 1");
+  if (channel < 0 || (int64_t)channel + nfreq > p->shape.nfrequencies)
+    return fail(BF_ERR_INVALID_ARGUMENT, "channels out of range");
   if (ntimes < 1) return fail(BF_ERR_INVALID_ARGUMENT, "ntimes must be >= 1");
   return BF_OK;
 }
@@ -192,24 +194,29 @@ int time_launches(hipStream_t st, int iters, float *ms, const std::function<int(
 
 extern "C" {
 
-int bf_form_beams_channel(const bf_plan *p, int channel, int64_t ntimes, uint8_t *J, const uint8_t *E, void *stream) {
-  if (int rc = check_channel_args(p, channel, ntimes, J, E)) return rc;
+int bf_form_beams_channels(const bf_plan *p, int channel, int nfreq, int64_t ntimes, uint8_t *J, const uint8_t *E,
+                           void *stream) {
+  if (int rc = check_channel_args(p, channel, nfreq, ntimes, J, E)) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const bf_shape &s = p->shape;
   const bool aligned = ((uintptr_t)E & 15u) == 0 && ((uintptr_t)J & 7u) == 0;
   if (p->use_mfma && aligned) {
     MfmaConfig c = p->cfg;
-    if (!choose_grid(ntimes, 1, (int64_t)p->num_cus * p->wgs_per_cu, &c))
+    if (!choose_grid(ntimes, nfreq, (int64_t)p->num_cus * p->wgs_per_cu, &c))
       return fail(BF_ERR_UNSUPPORTED, "ntimes too large");
-    return launch_mfma(p, c, J, E, ntimes, 1, channel, st);
+    return launch_mfma(p, c, J, E, ntimes, nfreq, channel, st);
   }
-  const int64_t total = s.nbeams * s.npolarizations * ntimes;
+  const int64_t total = s.nbeams * nfreq * s.npolarizations * ntimes;
   const int blocks = (int)std::min<int64_t>((total + 255) / 256, 1 << 20);
   hipLaunchKernelGGL(form_beams_generic_kernel, dim3(blocks), dim3(256), 0, st, E, J,
                      p->A_dev + (int64_t)channel * s.nbeams * s.ndishes, p->G_dev + (int64_t)channel * s.nbeams,
-                     ntimes, 1, (int)s.ndishes, (int)s.npolarizations, (int)s.nbeams, s.j_layout);
+                     ntimes, nfreq, (int)s.ndishes, (int)s.npolarizations, (int)s.nbeams, s.j_layout);
   BF_HIP(hipGetLastError());
   return BF_OK;
+}
+
+int bf_form_beams_channel(const bf_plan *p, int channel, int64_t ntimes, uint8_t *J, const uint8_t *E, void *stream) {
+  return bf_form_beams_channels(p, channel, 1, ntimes, J, E, stream);
 }
 
 int bf_batch_create(const bf_plan *p, int max_chunks, bf_batch **out) {

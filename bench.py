@@ -71,11 +71,200 @@ def cpu_baseline(E_host, A, G, T, F, D, B, f_sample, J_gpu):
         "cores": min(threads, B),
         "threads_requested": threads,
         "kind": "port",
-        "sample": f"oracle form_beams on T={T} F={f_sample} (of {F}) D={D} P=2 B={B}, same inputs; "
+        "sample": f"oracle form_beams on T={T} F={f_sample} (of {F}{', the whole workload' if f_sample == F else ''}) "
+                  f"D={D} P=2 B={B}, same inputs; "
                   f"{dt:.2f} s wall; OpenMP over beams (parallelism capped at B={B})",
         "seconds": dt,
         "beam_samples_per_s": B * f_sample * T * 2 / dt,
         "parity_vs_gpu": parity,
+    }
+
+
+def timed(stream, steps, warmup, fn):
+    """Mean milliseconds of fn() over `steps` calls after `warmup` calls, with
+    HIP events on `stream` and a device synchronisation on both sides."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        fn()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / steps
+
+
+def aligned_offsets(sizes, align=256):
+    offs, total = [], 0
+    for n in sizes:
+        offs.append(total)
+        total += (int(n) + align - 1) // align * align
+    return offs, total
+
+
+def run_batched(bf, bo, torch, stream, steps, warmup, seed=7, C=64, D=512, B=96):
+    """64 channels, each with its own ntimes drawn log-uniform in [512, 32768]:
+    one persistent launch (plain and hipGraph replay) against one
+    form_beams launch per chunk on the same stream."""
+    sptr = stream.cuda_stream
+    rng = np.random.default_rng(seed)
+    Ts = [int(x) for x in np.rint(np.exp(rng.uniform(np.log(512), np.log(32768), C)))]
+    A = bo.fill_uniform(C * B * D, seed + 1)
+    G = bo.setup_gain(C * B, D)
+    e_off, e_total = aligned_offsets([T * D * 2 for T in Ts])
+    j_off, j_total = aligned_offsets([B * 2 * T for T in Ts])
+    dE = torch.empty(e_total, dtype=torch.uint8, device="cuda")
+    bf.fill_uniform_device(dE.data_ptr(), e_total, seed + 2, 0, sptr)
+    dJb = torch.empty(j_total, dtype=torch.uint8, device="cuda")
+    dJp = torch.empty(j_total, dtype=torch.uint8, device="cuda")
+    plan = bf.Plan(bf.Shape.make(1, C, D, B), A, G)
+    batch = bf.Batch(plan, C)
+    chunks = [(dE.data_ptr() + e_off[c], dJb.data_ptr() + j_off[c], Ts[c], c) for c in range(C)]
+    tune = batch.autotune(chunks, sptr, iters=3)
+
+    def plain():
+        for c in range(C):
+            plan.form_beams_channel(c, Ts[c], dJp.data_ptr() + j_off[c], dE.data_ptr() + e_off[c], sptr)
+
+    ms_plain = timed(stream, steps, warmup, plain)
+    ms_batch = timed(stream, steps, warmup, lambda: batch.run(chunks, sptr))
+    ms_graph = timed(stream, steps, warmup, lambda: batch.run(chunks, sptr, graph=True))
+    same = bool(torch.equal(dJb, dJp))
+    # the oracle on the two shortest chunks
+    oracle_ok = True
+    for c in sorted(range(C), key=lambda c: Ts[c])[:2]:
+        E = dE[e_off[c]:e_off[c] + Ts[c] * D * 2].cpu().numpy()
+        want = bo.form_beams(E, A.reshape(C, -1)[c], G.reshape(C, -1)[c], Ts[c], 1, D, B)
+        oracle_ok &= bool(np.array_equal(dJb[j_off[c]:j_off[c] + B * 2 * Ts[c]].cpu().numpy(), want))
+    sumT = sum(Ts)
+    ops = workload_ops(sumT, 1, D, B)
+    best = min(ms_batch, ms_graph)
+    res = {
+        "workload": f"batched: {C} channels, D={D}, B={B}, P=2, ntimes log-uniform [512, 32768] (seed {seed}), "
+                    f"sum ntimes={sumT}",
+        "tops": round(ops / (best * 1e-3) / 1e12, 2),
+        "roofline_frac_mfma": round(ops / (best * 1e-3) / 1e12 / MI355X_INT8_TOPS, 4),
+        "hbm_gbps_algorithmic": round((sumT * D * 2 + C * B * D + 4 * C * B + sumT * B * 2) / (best * 1e-3) / 1e9, 1),
+        "beam_samples_per_s": sumT * B * 2 / (best * 1e-3),
+        "ms_persistent": round(ms_batch, 4),
+        "ms_graph": round(ms_graph, 4),
+        "ms_plain_per_chunk": round(ms_plain, 4),
+        "speedup_vs_plain": round(ms_plain / best, 3),
+        "bit_exact_vs_plain": same,
+        "oracle_checked_chunks": 2,
+        "oracle_ok": oracle_ok,
+        "batch": batch.describe(),
+        "autotune": tune.strip().splitlines()[-1],
+    }
+    batch.close()
+    plan.close()
+    return res
+
+
+def run_serving(bf, bo, torch, stream, seconds=2.0, rate=2000.0, seed=9, C=64, D=512, B=96,
+                budget=8 << 30, saturation=1000):
+    """Poisson arrivals (seeded, `rate` chunks/s) of chunks with nfreq in
+    {1, 4, 16} and ntimes uniform in [1024, 16384] into the server (weights
+    resident, HBM-budget admission); latency is submit -> completion seen by
+    the server. Then `saturation` requests submitted at once, against the
+    same requests as plain launches, one per request, on one stream."""
+    import threading
+    sptr = stream.cuda_stream
+    rng = np.random.default_rng(seed)
+    A = bo.fill_uniform(C * B * D, seed + 1)
+    G = bo.setup_gain(C * B, D)
+    pool_bytes = 1 << 30
+    pool = torch.empty(pool_bytes, dtype=torch.uint8, device="cuda")
+    bf.fill_uniform_device(pool.data_ptr(), pool_bytes, seed + 2, 0, sptr)
+    torch.cuda.synchronize()
+    n = int(seconds * rate)
+    gaps = rng.exponential(1.0 / rate, n)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    Ts = rng.integers(1024, 16385, n)
+    nfs = rng.choice([1, 4, 16], n)
+    chs = np.array([rng.integers(0, C - nf + 1) for nf in nfs])
+    ebytes = Ts * nfs * D * 2
+    offs = np.array([rng.integers(0, (pool_bytes - e) // 256 + 1) * 256 for e in ebytes])
+    reqs = [(pool.data_ptr() + int(offs[i]), int(Ts[i]), int(chs[i]), int(nfs[i]), i) for i in range(n)]
+    plan = bf.Plan(bf.Shape.make(1, C, D, B), A, G)
+    srv = bf.Server(plan, hbm_budget=budget)
+    results = {}
+
+    def consume(count):
+        while len(results) < count:
+            r = srv.wait(5.0)
+            if r is None:
+                raise RuntimeError(f"server stalled: {srv.stats()}")
+            if r.status != 0:
+                raise RuntimeError(f"request {r.tag} failed: status {r.status}")
+            results[r.tag] = (r.t_submit, r.t_done, r.batch_size)
+            srv.release(r.ticket)
+
+    # warm-up (graph-free path, kernels loaded, arena touched)
+    srv.submit_many(reqs[:32])
+    consume(32)
+    results.clear()
+
+    th = threading.Thread(target=consume, args=(n,))
+    th.start()
+    t0 = time.perf_counter()
+    late = []
+    for i, q in enumerate(reqs):
+        due = t0 + arrivals[i]
+        while True:
+            now = time.perf_counter()
+            if now >= due:
+                break
+            if due - now > 2e-4:
+                time.sleep(5e-5)
+        late.append(time.perf_counter() - due)
+        srv.submit(q[0], q[1], q[2], q[3], tag=q[4])
+    th.join()
+    lat = np.array([results[i][1] - results[i][0] for i in range(n)])
+    t_first = min(v[0] for v in results.values())
+    t_last = max(v[1] for v in results.values())
+    bsamp = float(np.sum(Ts * nfs) * B * 2)
+    st = srv.stats()
+
+    # saturation: all at once, then the same requests one launch each
+    sat = reqs[:saturation]
+    sat_bs = float(np.sum(Ts[:saturation] * nfs[:saturation]) * B * 2)
+    results.clear()
+    torch.cuda.synchronize()
+    c0 = srv.clock()
+    srv.submit_many(sat)
+    consume(len(sat))
+    sat_s = max(v[1] for v in results.values()) - c0
+    jbuf = torch.empty(int(max(B * 2 * Ts[:saturation] * nfs[:saturation])), dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    for q in sat:
+        plan.form_beams_channels(q[2], q[3], q[1], jbuf.data_ptr(), q[0], sptr)
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - tp
+    srv.close()
+    plan.close()
+    return {
+        "workload": f"serving: Poisson arrivals {rate:.0f} chunks/s for {seconds:.1f} s (seed {seed}), nfreq in "
+                    f"{{1,4,16}}, ntimes uniform [1024, 16384], D={D}, B={B}, P=2, {C}-channel table resident, "
+                    f"E resident in HBM, HBM budget {budget >> 20} MiB",
+        "requests": n,
+        "p50_latency_ms": round(float(np.percentile(lat, 50)) * 1e3, 4),
+        "p99_latency_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),
+        "max_latency_ms": round(float(lat.max()) * 1e3, 4),
+        "sustained_beam_samples_per_s": bsamp / (t_last - t_first),
+        "offered_chunks_per_s": rate,
+        "client_lateness_p99_ms": round(float(np.percentile(late, 99)) * 1e3, 4),
+        "launches": int(st["launches"]),
+        "mean_requests_per_launch": round(float(st["mean_batch"]), 3),
+        "peak_arena_bytes": int(st["peak_bytes"]),
+        "saturation_requests": len(sat),
+        "saturation_beam_samples_per_s": sat_bs / sat_s,
+        "plain_launch_beam_samples_per_s": sat_bs / plain_s,
+        "saturation_speedup_vs_plain_launch": round(plain_s / sat_s, 3),
     }
 
 
@@ -88,8 +277,12 @@ def main():
     ap.add_argument("--nfrequencies", type=int, default=32)
     ap.add_argument("--ndishes", type=int, default=512)
     ap.add_argument("--nbeams", type=int, default=96)
-    ap.add_argument("--cpu-frequencies", type=int, default=4, help="frequencies in the CPU baseline sample")
+    ap.add_argument("--cpu-frequencies", type=int, default=32, help="frequencies in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--full-only", action="store_true", help="skip the batched and serving workloads")
+    ap.add_argument("--autotune", action="store_true",
+                    help="autotune the full-size kernel configuration before timing (default: the built-in choice)")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--copy-calib", type=int, default=0, help="only run a device copy of this many bytes")
     args = ap.parse_args()
 
@@ -125,27 +318,30 @@ def main():
     bf.fill_uniform_device(dE.data_ptr(), shape.e_bytes, 11, 0, sptr)
     dJ = torch.empty(shape.j_bytes, dtype=torch.uint8, device=dev)
     plan = bf.Plan(shape, A, G)
-    log(f"kernel: {plan.describe()}")
+    log(f"kernel (default): {plan.describe()}")
+    tune = None
+    if args.autotune:
+        tune = plan.autotune(dJ.data_ptr(), dE.data_ptr(), sptr, iters=5)
+        log(tune)
+        log(f"kernel (tuned): {plan.describe()}")
 
-    for _ in range(args.warmup):
-        plan.form_beams_device(dJ.data_ptr(), dE.data_ptr(), sptr)
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        plan.form_beams_device(dJ.data_ptr(), dE.data_ptr(), sptr)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = ev0.elapsed_time(ev1) / args.steps
+    ms = timed(stream, args.steps, args.warmup, lambda: plan.form_beams_device(dJ.data_ptr(), dE.data_ptr(), sptr))
+    wall = time.perf_counter() - t0  # warm-up and timed steps
 
     ops = workload_ops(T, F, D, B)
     nbytes = workload_bytes(T, F, D, B)
     tops = ops / (ms * 1e-3) / 1e12
     gbps = nbytes / (ms * 1e-3) / 1e9
     checksum = bf.adler32_device(dJ.data_ptr(), shape.j_bytes, sptr)
+    log(f"full: {ms * 1e3:.1f} us/launch, {tops:.1f} TOPS")
+
+    batched = serving = None
+    if not args.full_only:
+        batched = run_batched(bf, bo, torch, stream, max(5, args.steps // 5), 3)
+        log(f"batched: {batched}")
+        serving = run_serving(bf, bo, torch, stream, seconds=args.serve_seconds)
+        log(f"serving: {serving}")
 
     cpu = None
     if not args.no_cpu:
@@ -173,13 +369,17 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(tops, 2), "peak": MI355X_INT8_TOPS, "unit": "TFLOP/s",
                      "frac": round(tops / MI355X_INT8_TOPS, 4), "traffic": traffic,
                      "note": "int8 MFMA ops per second vs the 5.0 POP/s dense int8 peak; achieved = algorithmic "
-                             "ops per launch / mean launch time (HIP events)"},
+                             "ops per launch / mean launch time (HIP events on the launch stream); traffic = "
+                             "HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated"},
         "beam_samples_per_s": B * F * T * 2 / (ms * 1e-3),
         "hbm_gbps_algorithmic": round(gbps, 1),
         "hbm_frac": round(gbps / MI355X_HBM_GBPS, 4),
         "kernel": plan.describe(),
+        "autotune": tune.strip().splitlines()[-1] if tune else None,
         "j_adler32": f"0x{checksum:08x}",
-        "wall_s_timed_region": round(wall, 4),
+        "wall_s_warmup_and_steps": round(wall, 4),
+        "batched_vs_plain_launch": batched,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     plan.close()

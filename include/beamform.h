@@ -144,6 +144,12 @@ int bf_fill_uniform_device(uint8_t *buf, size_t n, uint64_t seed, uint64_t offse
 int bf_form_beams_channel(const bf_plan *plan, int channel, int64_t ntimes, uint8_t *J, const uint8_t *E,
                           void *stream);
 
+/* Same for nfreq consecutive channels channel .. channel + nfreq - 1 in one
+ * launch: E is [ntimes][nfreq][D][P], J is [B][nfreq][P][ntimes] (or
+ * [B][nfreq][ntimes][P]). */
+int bf_form_beams_channels(const bf_plan *plan, int channel, int nfreq, int64_t ntimes, uint8_t *J,
+                           const uint8_t *E, void *stream);
+
 typedef struct bf_chunk {
   const uint8_t *E; /* device: [ntimes][nfreq][D][P] (Elinear with F = nfreq) */
   uint8_t *J;       /* device: [B][nfreq][P][ntimes] or [B][nfreq][ntimes][P] per the plan layout */

@@ -174,3 +174,27 @@ def test_plan_autotune_keeps_results_exact():
         torch.cuda.synchronize()
         say(plan.describe())
     assert np.array_equal(dJ.cpu().numpy(), bo.form_beams(E, A, G, T, F, D, B))
+
+
+@pytest.mark.parametrize("layout", [bf.BF_J_BFPT, bf.BF_J_BFTP])
+def test_multi_channel_plain_and_batched(layout):
+    C, D, B = 20, 512, 96
+    A, G = channel_table(C, D, B, 15)
+    shape = bf.Shape.make(1, C, D, B, j_layout=layout)
+    specs = [(1000, 3, 4), (17, 0, 16), (2500, 19, 1), (64, 4, 16)]  # (ntimes, channel, nfreq)
+    host, dEs, dJp, dJb = [], [], [], []
+    for i, (T, c, nf) in enumerate(specs):
+        E = bo.fill_uniform(T * nf * D * 2, 600 + i)
+        host.append(E)
+        dEs.append(torch.from_numpy(E).cuda())
+        dJp.append(torch.zeros(B * nf * 2 * T, dtype=torch.uint8, device="cuda"))
+        dJb.append(torch.zeros(B * nf * 2 * T, dtype=torch.uint8, device="cuda"))
+    with bf.Plan(shape, A, G) as plan, bf.Batch(plan, 8) as batch:
+        for (T, c, nf), dE, dJ in zip(specs, dEs, dJp):
+            plan.form_beams_channels(c, nf, T, dJ.data_ptr(), dE.data_ptr(), stream())
+        batch.run([(dE.data_ptr(), dJ.data_ptr(), T, c, nf) for (T, c, nf), dE, dJ in zip(specs, dEs, dJb)], stream())
+        torch.cuda.synchronize()
+    for i, ((T, c, nf), E) in enumerate(zip(specs, host)):
+        want = bo.form_beams(E, A.reshape(C, -1)[c:c + nf], G.reshape(C, -1)[c:c + nf], T, nf, D, B, jlayout=layout)
+        assert np.array_equal(dJp[i].cpu().numpy(), want), f"plain launch, chunk {i}"
+        assert np.array_equal(dJb[i].cpu().numpy(), want), f"batched, chunk {i}"
