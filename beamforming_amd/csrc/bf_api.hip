@@ -134,8 +134,7 @@ bool choose_tiling(int64_t ndishes, int64_t nbeams, int forced_nt, MfmaConfig *c
   cfg->nt = best_nt;
   cfg->nbg = best_nbg;
   cfg->ntp = best_nt * best_nbg;
-  cfg->lds_bytes = (size_t)ks * best_nt * 64 * sizeof(v4i) + (size_t)best_nt * 32 * (sizeof(int) * 2 + sizeof(float)) +
-                   (size_t)kWaves * 64 * sizeof(v4i);
+  cfg->lds_bytes = mfma_lds_bytes(best_nt, ks);
   cfg->dish_tail = (int64_t)ks * kDishesPerStep != ndishes;
   cfg->kernel = lookup_mfma(best_nt, ks, cfg->dish_tail);
   cfg->batch_kernel = lookup_batch(best_nt, ks, cfg->dish_tail);

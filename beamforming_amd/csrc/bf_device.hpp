@@ -44,7 +44,8 @@ BF_HD uint32_t enc(int re, int im) { return ((((uint32_t)re & 0xfu) << 4) | ((ui
 // variants as overlapping register windows: v2 = u[0..3], v1 = u[2..5].
 
 // Byte permute: result byte i = byte sel_i of the 8-byte value (hi:lo),
-// selectors 0-3 address lo, 4-7 address hi (v_perm_b32).
+// selectors 0-3 address lo, 4-7 address hi, 12 gives 0x00 (v_perm_b32).
+// Other selector values are not used here.
 BF_HD uint32_t perm_bytes(uint32_t hi, uint32_t lo, uint32_t sel) {
 #if defined(__HIP_DEVICE_COMPILE__)
   return __builtin_amdgcn_perm(hi, lo, sel);
@@ -53,7 +54,8 @@ BF_HD uint32_t perm_bytes(uint32_t hi, uint32_t lo, uint32_t sel) {
   uint32_t r = 0;
   for (int i = 0; i < 4; ++i) {
     const uint32_t s = (sel >> (8 * i)) & 0xffu;
-    r |= (uint32_t)((v >> (8 * s)) & 0xffu) << (8 * i);
+    const uint32_t byte = s < 8 ? (uint32_t)((v >> (8 * s)) & 0xffu) : 0u;
+    r |= byte << (8 * i);
   }
   return r;
 #endif
@@ -121,17 +123,32 @@ BF_HD float requant_exact(float g, int raw) {
   return (float)q;
 }
 
-// Encoded J byte as a float: enc(qr, qi) = 16 (qr + 8) + (qi + 8) for
-// qr, qi in [-7, 7]; every step is exact in float.
-BF_HD float jbyte_f(float qr, float qi) { return fmaf(qr, 16.0f, qi) + 136.0f; }
-
-// Inserts the byte value v (an exact integer 0..255 as float) at byte pos.
-BF_HD uint32_t put_byte(float v, int pos, uint32_t word) {
+// Fast requantisation with the result in integer bits: the low byte of the
+// returned word is requant_fast(g, raw) + 8, a nibble 1..15 (the ucomplex4
+// offset-binary digit). c = clamp(g * raw, -7, 7) is added to
+// M = 1.5 * 2^23 + 8: floats in [2^23, 2^24) are the integers, so the sum is
+// rounded to the nearest integer, ties to even; M is even, so the tie goes
+// to the even rint(c), as in rint. The float bits of M + 8 + rint(c) are
+// bits(1.5 * 2^23) + 8 + rint(c), whose low 22 bits are 8 + rint(c).
+BF_HD uint32_t requant_bits(float g, int raw) {
+  const float y = g * (float)raw;
 #if defined(__HIP_DEVICE_COMPILE__)
-  return __builtin_amdgcn_cvt_pk_u8_f32(v, pos, word);
+  const float c = __builtin_amdgcn_fmed3f(y, -7.0f, 7.0f);
 #else
-  return (word & ~(0xffu << (8 * pos))) | ((uint32_t)(int)v << (8 * pos));
+  const float c = fminf(fmaxf(y, -7.0f), 7.0f);
 #endif
+  return __builtin_bit_cast(uint32_t, c + 12582920.0f);
+}
+
+// ucomplex4 byte of requant_bits pairs in the low byte: (re + 8) << 4 |
+// (im + 8), i.e. enc(qr, qi) (icomplex4.hxx:33). Higher bytes are garbage.
+BF_HD uint32_t jbyte_bits(uint32_t re_bits, uint32_t im_bits) { return (re_bits << 4) | im_bits; }
+
+// Word of the low bytes of b0..b3 (byte i = low byte of bi).
+BF_HD uint32_t pack_low_bytes(uint32_t b0, uint32_t b1, uint32_t b2, uint32_t b3) {
+  const uint32_t t01 = perm_bytes(b1, b0, 0x0c0c0400u);
+  const uint32_t t23 = perm_bytes(b3, b2, 0x0c0c0400u);
+  return perm_bytes(t23, t01, 0x05040100u);
 }
 
 // Offsets that the E-side nibble encoding adds to the two accumulators.

@@ -83,7 +83,7 @@ __global__ void form_beams_generic_kernel(const uint8_t *__restrict__ E, uint8_t
     const float g = G[(int64_t)f * B + b];
     const float qr = requant_exact(g, rr), qi = requant_exact(g, ri);
     const int64_t off = jlayout == 0 ? (((int64_t)b * F + f) * P + p) * T + t : (((int64_t)b * F + f) * T + t) * P + p;
-    J[off] = (uint8_t)(int)jbyte_f(qr, qi);
+    J[off] = (uint8_t)enc((int)qr, (int)qi);
   }
 }
 

@@ -18,10 +18,15 @@
 //    MFMAs (real and imaginary accumulator);
 //  * the int32 accumulators start at minus the per-beam nibble offsets, so
 //    they end at the exact complex sums of cpu.cxx:24-35; the fused epilogue
-//    applies the float gain, round-to-nearest-even and the +-7 clamp
-//    (cpu.cxx:37-39), packs ucomplex4 bytes, swaps halves between lanes l and
-//    l ^ 32 so that each lane holds 16 consecutive times of one
-//    (beam, polarization) row, and stores 16 bytes per lane.
+//    applies the float gain, the +-7 clamp and round-to-nearest-even
+//    (cpu.cxx:37-39), packs ucomplex4 bytes and swaps halves between lanes
+//    l and l + 32 (v_permlane32_swap), so that each lane holds the 16 bytes
+//    of 16 consecutive times of one (beam, polarization) row;
+//  * the 8 waves advance in rounds of 8 consecutive tiles. With the
+//    [b][f][p][t] layout and room in LDS, each wave writes its rows into a
+//    double-buffered LDS staging area and, after the round's barrier, the
+//    workgroup stores every row's 128 bytes (8 tiles) with whole-line
+//    writes; otherwise each lane stores its 16 bytes directly.
 // Rows past T and dishes past D are clamped, always-in-bounds loads whose
 // values are replaced by zeros; the weights there are zero too. The plan
 // routes gains outside the fast requantisation range (non-finite, or
@@ -69,14 +74,21 @@ __device__ __forceinline__ void store_row8(uint8_t *dst, uint32_t w0, uint32_t w
 }
 
 // LDS of a workgroup: [KS][NT][64] weight operands, the beam-column nibble
-// offsets and gains of the beam group, then one 1 KiB E transpose buffer per
-// wave.
-template <int NT, int KS> struct LdsPlan {
-  static constexpr size_t weights = (size_t)KS * NT * 64 * sizeof(v4i);
-  static constexpr size_t beams = (size_t)NT * 32 * (sizeof(int2) + sizeof(float));
-  static constexpr size_t transpose = (size_t)kWaves * 64 * sizeof(v4i);
-  static constexpr size_t bytes = weights + beams + transpose;
-};
+// offsets and gains of the beam group, one 1 KiB E transpose buffer per wave,
+// then, when it fits in the 160 KiB of a CU, the J staging area: 2 rounds x
+// (NT x 32 beams x 2 polarizations) rows x 128 bytes.
+constexpr size_t kLdsPerCu = 160 * 1024;
+BF_HD constexpr size_t mfma_lds_unstaged(int nt, int ks) {
+  return (size_t)ks * nt * 64 * sizeof(v4i) + (size_t)nt * 32 * (sizeof(int2) + sizeof(float)) +
+         (size_t)kWaves * 64 * sizeof(v4i);
+}
+BF_HD constexpr size_t mfma_lds_staging(int nt) { return (size_t)2 * nt * 64 * 128; }
+BF_HD constexpr bool mfma_stages_j(int nt, int ks) {
+  return mfma_lds_unstaged(nt, ks) + mfma_lds_staging(nt) <= kLdsPerCu;
+}
+BF_HD constexpr size_t mfma_lds_bytes(int nt, int ks) {
+  return mfma_lds_unstaged(nt, ks) + (mfma_stages_j(nt, ks) ? mfma_lds_staging(nt) : 0);
+}
 
 // Stages row `wrow` (a frequency, or a channel of a channel table) of the
 // prepared weights, beam group bg, into LDS. All threads of the workgroup
@@ -109,19 +121,36 @@ struct TileJob {
   int32_t jlayout;   // 0 = [b][f][p][t], 1 = [b][f][t][p]
 };
 
-// The per-wave pipeline: tiles tile, tile + step, ... < tile_end of one job,
-// against the weights staged in LDS (Wl, Cl, Gl), using the wave's transpose
-// buffer xw.
+// Stores the 16 bytes (v.x .. v.w, little-endian) at dst. fast: dst is
+// 16-byte aligned and all 16 bytes are inside the row.
+__device__ __forceinline__ void store_piece(uint8_t *dst, const v4i v, int n, bool fast) {
+  if (fast)
+    *reinterpret_cast<v4i *>(dst) = v;
+  else
+    store_row16(dst, (uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z, (uint32_t)v.w, n);
+}
+
+// The tile pipeline of a workgroup: tiles tile_first .. tile_end - 1 of one
+// job, in rounds of kWaves consecutive tiles (tile tile_first + kWaves r +
+// wave in round r), against the weights staged in LDS (Wl, Cl, Gl), using
+// the wave's transpose buffer xw. Sl is the J staging area (NULL: direct
+// stores). All waves of the workgroup call this with the same arguments.
 template <int NT, int KS, bool DT>
-__device__ __forceinline__ void run_tiles(const TileJob &job, int64_t tile, const int64_t tile_end, const int64_t step,
-                                          const v4i *Wl, const int2 *Cl, const float *Gl, v4i *xw) {
+__device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile_first, const int64_t tile_end,
+                                          const v4i *Wl, const int2 *Cl, const float *Gl, v4i *xw, v4i *Sl) {
   static_assert(NT >= 1 && NT <= 4, "32-beam tiles per workgroup");
   static_assert(KS >= 2 && KS <= 64 && KS % 2 == 0, "K steps (pairs)");
   constexpr int KP = KS / 2;                  // K-step pairs (one E load each)
-  constexpr int RP0 = NT >= 4 ? 8 : 16;       // E prefetch distance in pairs
+  constexpr int RP0 = 8;                      // E prefetch distance in pairs
   constexpr int RP = RP0 < KP ? RP0 : KP;
   static_assert(KP % RP == 0, "ring slots must line up across tiles");
-  if (tile >= tile_end) return;
+  constexpr int64_t step = kWaves;
+  const int wave = threadIdx.x >> 6;
+  const int64_t rounds = tile_end > tile_first ? (tile_end - tile_first + step - 1) / step : 0;
+  int64_t tile = tile_first + wave;
+  // Whole-piece 16-byte stores: J rows start 16-byte aligned and every piece
+  // is complete (uniform).
+  const bool fast_store = ((uintptr_t)job.J & 15u) == 0 && (job.T & 15) == 0;
 
   const int lane = threadIdx.x & 63;
   const int h = lane >> 5;          // k half (operands); row half (accumulators)
@@ -167,7 +196,8 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, int64_t tile, cons
     for (int q = 0; q < RP; ++q) ring[q] = load_pair(p0, q);
   }
 
-  for (; tile < tile_end; tile += step) {
+  for (int64_t r = 0; r < rounds; ++r, tile += step) {
+  if (tile < tile_end) { // wave-uniform
     v16i acc[NT][2];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
@@ -251,51 +281,93 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, int64_t tile, cons
       const float gain = Gl[nt * 32 + col];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        uint32_t w = 0;
+        uint32_t b[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int j = 4 * k + i;
-          const float qr = requant_fast(gain, acc[nt][0][j]);
-          const float qi = requant_fast(gain, acc[nt][1][j]);
-          w = put_byte(jbyte_f(qr, qi), i, w);
-        }
-        wv[nt][k] = w;
+        for (int i = 0; i < 4; ++i)
+          b[i] = jbyte_bits(requant_bits(gain, acc[nt][0][4 * k + i]), requant_bits(gain, acc[nt][1][4 * k + i]));
+        wv[nt][k] = pack_low_bytes(b[0], b[1], b[2], b[3]);
       }
     }
-    // The E loads of the next tile's first steps were issued thousands of
-    // cycles ago; completing them here, before the stores, keeps the stores
-    // (counted by the same vmcnt) out of every wait of the next tile.
-    __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0)
+    if (job.jlayout == 0) {
+      // J[b][f][p][t]: v_permlane32_swap exchanges lanes 32..63 of its first
+      // operand with lanes 0..31 of its second. Swapping (p0 t0-3 | p1 t0-3)
+      // pairs and (p0 t8-11 | p1 t8-11) pairs leaves lane h = 0 with the
+      // 16 bytes of polarization 0 and lane h = 1 with those of polarization
+      // 1, in the same register order.
+      v4i row[NT];
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-      const int b = (job.bg * NT + nt) * 32 + col;
-      if (job.jlayout == 0) {
-        // J[b][f][p][t]: lane h = 0 assembles polarization 0, lane h = 1
-        // polarization 1, exchanging the other halves with lane ^ 32.
-        const uint32_t x = h ? wv[nt][0] : wv[nt][2];
-        const uint32_t y = h ? wv[nt][1] : wv[nt][3];
-        const uint32_t xr = (uint32_t)__shfl_xor((int)x, 32, 64);
-        const uint32_t yr = (uint32_t)__shfl_xor((int)y, 32, 64);
-        if (b < job.B) {
-          uint8_t *dst = job.J + (((int64_t)b * job.F + job.f) * 2 + h) * job.T + t0;
-          if (h == 0)
-            store_row16(dst, wv[nt][0], xr, wv[nt][1], yr, nvalid);
-          else
-            store_row16(dst, xr, wv[nt][2], yr, wv[nt][3], nvalid);
+      for (int nt = 0; nt < NT; ++nt) {
+        const auto a = __builtin_amdgcn_permlane32_swap(wv[nt][0], wv[nt][2], false, false);
+        const auto c = __builtin_amdgcn_permlane32_swap(wv[nt][1], wv[nt][3], false, false);
+        row[nt] = v4i{(int)a[0], (int)a[1], (int)c[0], (int)c[1]};
+      }
+      if (Sl) {
+        // Row (beam nt * 32 + col, polarization h), 16-byte piece `wave`
+        // of the round's 128 bytes; piece slots are rotated by the row so
+        // that the 8 rows of a lane octet hit distinct banks.
+        v4i *buf = Sl + (size_t)(r & 1) * (NT * 64 * 8);
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          const int srow = (nt * 32 + col) * 2 + h;
+          buf[srow * 8 + (wave ^ ((srow >> 1) & 7))] = row[nt];
         }
-      } else if (b < job.B) {
-        // J[b][f][t][p]: interleave the polarizations of 4 times per half.
-        uint8_t *row = job.J + (((int64_t)b * job.F + job.f) * job.T + t0) * 2;
+      } else {
+        // The E loads of the next tile's first steps were issued thousands
+        // of cycles ago; completing them here keeps the stores (counted by
+        // the same vmcnt) out of every wait of the next tile.
+        __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0)
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-          const int tt = 8 * k + 4 * h; // first time of this group in the tile
-          const uint32_t lo = perm_bytes(wv[nt][2 + k], wv[nt][k], 0x05010400u);
-          const uint32_t hi = perm_bytes(wv[nt][2 + k], wv[nt][k], 0x07030602u);
-          store_row8(row + 2 * tt, lo, hi, 2 * (nvalid - tt));
+        for (int nt = 0; nt < NT; ++nt) {
+          const int b = (job.bg * NT + nt) * 32 + col;
+          if (b < job.B)
+            store_piece(job.J + (((int64_t)b * job.F + job.f) * 2 + h) * job.T + t0, row[nt], nvalid, fast_store);
         }
       }
+    } else {
+      __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0), as above
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const int b = (job.bg * NT + nt) * 32 + col;
+        if (b < job.B) {
+          // J[b][f][t][p]: interleave the polarizations of 4 times per half.
+          uint8_t *row = job.J + (((int64_t)b * job.F + job.f) * job.T + t0) * 2;
+#pragma unroll
+          for (int k = 0; k < 2; ++k) {
+            const int tt = 8 * k + 4 * h; // first time of this group in the tile
+            const uint32_t lo = perm_bytes(wv[nt][2 + k], wv[nt][k], 0x05010400u);
+            const uint32_t hi = perm_bytes(wv[nt][2 + k], wv[nt][k], 0x07030602u);
+            store_row8(row + 2 * tt, lo, hi, 2 * (nvalid - tt));
+          }
+        }
+      }
+    }
+  } // active tile
+
+  if (Sl) {
+    // The round's rows are complete once every wave has passed the barrier
+    // (its LDS writes done). Thread q writes piece q & 7 of staging row
+    // q >> 3: 8 consecutive threads store one row's 128 contiguous bytes.
+    // The other buffer is rewritten only after the next round's barrier,
+    // which every wave reaches after finishing these reads.
+    __builtin_amdgcn_s_waitcnt(0xc07f); // lgkmcnt(0)
+    __builtin_amdgcn_s_barrier();
+    const v4i *buf = Sl + (size_t)(r & 1) * (NT * 64 * 8);
+    const int64_t t_round = (tile_first + r * step) * kTileT;
+    const int64_t t_end = tile_end * kTileT < job.T ? tile_end * kTileT : job.T;
+    __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0), as above
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+      const int q = threadIdx.x + kThreads * k;
+      const int srow = q >> 3, seg = q & 7;
+      const v4i v = buf[srow * 8 + (seg ^ ((srow >> 1) & 7))];
+      const int b = job.bg * NT * 32 + (srow >> 1);
+      const int64_t t = t_round + 16 * seg;
+      if (b < job.B && t < t_end)
+        store_piece(job.J + (((int64_t)b * job.F + job.f) * 2 + (srow & 1)) * job.T + t, v,
+                    t_end - t < 16 ? (int)(t_end - t) : 16, fast_store);
     }
   }
+  } // rounds
 }
 
 template <int NT, int KS, bool DT>
@@ -311,7 +383,7 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
   stage_weights<NT, KS>(a.W, a.corr, a.G, a.NTP, a.wrow0 + wi.f, wi.bg, Wl, Cl, Gl);
   __syncthreads();
 
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t ntiles = (a.T + kTileT - 1) / kTileT;
   const int64_t tile_begin = (int64_t)wi.chunk * a.tiles_per_chunk;
   const int64_t tile_end = ntiles < tile_begin + a.tiles_per_chunk ? ntiles : tile_begin + a.tiles_per_chunk;
@@ -326,7 +398,8 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
   job.B = a.B;
   job.bg = wi.bg;
   job.jlayout = a.jlayout;
-  run_tiles<NT, KS, DT>(job, tile_begin + wave, tile_end, kWaves, Wl, Cl, Gl, Xl + wave * 64);
+  v4i *Sl = mfma_stages_j(NT, KS) && a.jlayout == 0 ? Xl + kWaves * 64 : nullptr;
+  run_tiles<NT, KS, DT>(job, tile_begin, tile_end, Wl, Cl, Gl, Xl + wave * 64, Sl);
 }
 
 // ---------------------------------------------------------------------------
@@ -366,13 +439,14 @@ __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a
   float *Gl = reinterpret_cast<float *>(Cl + NT * 32);
   v4i *Xl = reinterpret_cast<v4i *>(Gl + NT * 32);
   __shared__ int s_item;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  v4i *Sl = mfma_stages_j(NT, KS) && a.jlayout == 0 ? Xl + kWaves * 64 : nullptr;
   int loaded_channel = -1, loaded_bg = -1;
   const int nitems = *a.nitems;
   for (;;) {
     if (threadIdx.x == 0) s_item = (int)atomicAdd(a.head, 1u);
     __syncthreads();
-    const int it = s_item;
+    const int it = __builtin_amdgcn_readfirstlane(s_item);
     if (it >= nitems) break; // uniform: every thread read the same s_item
     const BatchItem item = a.items[it];
     const BatchChunk ch = a.chunks[item.chunk];
@@ -394,7 +468,7 @@ __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a
     job.B = a.B;
     job.bg = item.bg;
     job.jlayout = a.jlayout;
-    run_tiles<NT, KS, DT>(job, item.tile_begin + wave, item.tile_end, kWaves, Wl, Cl, Gl, Xl + wave * 64);
+    run_tiles<NT, KS, DT>(job, item.tile_begin, item.tile_end, Wl, Cl, Gl, Xl + wave * 64, Sl);
     __syncthreads(); // LDS and s_item are reused by the next item
   }
 }
