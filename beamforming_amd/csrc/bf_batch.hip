@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <functional>
+#include <queue>
 #include <string>
 #include <vector>
 
@@ -78,12 +79,51 @@ int ensure_item_capacity(bf_batch *b, int n) {
   return BF_OK;
 }
 
-// Default item size: about two items per resident workgroup, a multiple of
-// the 8 waves, between 8 and 256 tiles.
-int default_item_tiles(int64_t total_tiles, int grid) {
-  int64_t t = (total_tiles + 2 * (int64_t)grid - 1) / (2 * (int64_t)grid);
-  t = (t + kWaves - 1) / kWaves * kWaves;
-  return (int)std::min<int64_t>(256, std::max<int64_t>(kWaves, t));
+int round_up_waves(int64_t t) { return (int)((t + kWaves - 1) / kWaves * kWaves); }
+
+// Work items with a fixed size of `it` tiles, in chunk order.
+void fixed_items(const bf_chunk *chunks, int n, int nbg, int it, std::vector<BatchItem> &items) {
+  for (int i = 0; i < n; ++i) {
+    const int64_t nt = (chunks[i].ntimes + kTileT - 1) / kTileT;
+    const int nf = chunk_nfreq(chunks[i]);
+    for (int f = 0; f < nf; ++f)
+      for (int bg = 0; bg < nbg; ++bg)
+        for (int64_t t = 0; t < nt; t += it)
+          items.push_back(BatchItem{i, (uint16_t)f, (uint16_t)bg, (int32_t)t, (int32_t)std::min(nt, t + it)});
+  }
+}
+
+// Guided self-scheduling: items are cut, largest first, from the longest
+// remaining piece of any (chunk, frequency, beam group), each about
+// remaining / (2 grid) tiles (a multiple of the 8 waves, between 32 and 256;
+// every item start costs one E pipeline fill, so the tail is not cut finer
+// than 4 rounds). Workgroups take them in this order from the queue, so the
+// big items are spread over the grid first and the shrinking tail fills in
+// the imbalance.
+void guided_items(const bf_chunk *chunks, int n, int nbg, int grid, int64_t total, std::vector<BatchItem> &items) {
+  struct Piece {
+    int64_t begin, end;
+    int chunk, f, bg;
+    bool operator<(const Piece &o) const { return end - begin < o.end - o.begin; }
+  };
+  std::priority_queue<Piece> heap;
+  for (int i = 0; i < n; ++i) {
+    const int64_t nt = (chunks[i].ntimes + kTileT - 1) / kTileT;
+    const int nf = chunk_nfreq(chunks[i]);
+    for (int f = 0; f < nf; ++f)
+      for (int bg = 0; bg < nbg; ++bg) heap.push(Piece{0, nt, i, f, bg});
+  }
+  int64_t remaining = total;
+  while (!heap.empty()) {
+    Piece p = heap.top();
+    heap.pop();
+    const int64_t want = std::min<int64_t>(256, std::max<int64_t>(4 * kWaves, round_up_waves(remaining / (2 * (int64_t)grid))));
+    const int64_t size = std::min(want, p.end - p.begin);
+    items.push_back(BatchItem{p.chunk, (uint16_t)p.f, (uint16_t)p.bg, (int32_t)p.begin, (int32_t)(p.begin + size)});
+    remaining -= size;
+    p.begin += size;
+    if (p.begin < p.end) heap.push(p);
+  }
 }
 
 // Validates the chunk list, builds the work items (longest first) into the
@@ -105,21 +145,19 @@ int upload(bf_batch *b, const bf_chunk *chunks, int n, hipStream_t st) {
       return fail(BF_ERR_INVALID_ARGUMENT, "chunk " + std::to_string(i) + ": E must be 16-byte and J 8-byte aligned");
     total_tiles += (k.ntimes + kTileT - 1) / kTileT * c.nbg * nf;
   }
-  const int it = b->item_tiles > 0 ? b->item_tiles : default_item_tiles(total_tiles, b->grid);
+  for (int i = 0; i < n; ++i)
+    if ((chunks[i].ntimes + kTileT - 1) / kTileT > (1ll << 30)) return fail(BF_ERR_UNSUPPORTED, "chunk too long");
   std::vector<BatchItem> items;
-  items.reserve((size_t)(total_tiles / it + (int64_t)n * c.nbg + 1));
-  for (int i = 0; i < n; ++i) {
-    const int64_t nt = (chunks[i].ntimes + kTileT - 1) / kTileT;
-    if (nt > (1ll << 30)) return fail(BF_ERR_UNSUPPORTED, "chunk too long");
-    const int nf = chunk_nfreq(chunks[i]);
-    for (int f = 0; f < nf; ++f)
-      for (int bg = 0; bg < c.nbg; ++bg)
-        for (int64_t t = 0; t < nt; t += it)
-          items.push_back(BatchItem{i, (uint16_t)f, (uint16_t)bg, (int32_t)t, (int32_t)std::min(nt, t + it)});
+  if (b->item_tiles > 0) {
+    fixed_items(chunks, n, c.nbg, b->item_tiles, items);
+    // longest first
+    std::stable_sort(items.begin(), items.end(), [](const BatchItem &x, const BatchItem &y) {
+      return x.tile_end - x.tile_begin > y.tile_end - y.tile_begin;
+    });
+  } else {
+    guided_items(chunks, n, c.nbg, b->grid, total_tiles, items);
   }
-  std::stable_sort(items.begin(), items.end(), [](const BatchItem &x, const BatchItem &y) {
-    return x.tile_end - x.tile_begin > y.tile_end - y.tile_begin;
-  });
+  const int it = items.empty() ? 0 : items.front().tile_end - items.front().tile_begin;
   if (int rc = ensure_item_capacity(b, std::max<int>(1, (int)items.size()))) return rc;
   if (b->upload_pending) { // the staging buffers may still be read by the previous copy
     BF_HIP(hipEventSynchronize(b->upload_done));
@@ -320,7 +358,7 @@ int bf_batch_describe(const bf_batch *b, char *buf, size_t len) {
   std::snprintf(buf, len,
                 "persistent mfma_i32_32x32x32_i8 batch: nt=%d ks=%d nbg=%d grid=%d lds=%zu item_tiles=%d%s items=%d "
                 "graph=%s",
-                c.nt, c.ks, c.nbg, b->grid, c.lds_bytes, b->last_item_tiles, b->item_tiles ? "" : " (auto)",
+                c.nt, c.ks, c.nbg, b->grid, c.lds_bytes, b->last_item_tiles, b->item_tiles ? "" : " (largest, guided)",
                 b->last_nitems, b->exec ? "captured" : "none");
   return BF_OK;
 }

@@ -27,8 +27,9 @@
 //    double-buffered LDS staging area and, after the round's barrier, the
 //    workgroup stores every row's 128 bytes (8 tiles) with whole-line
 //    writes; otherwise each lane stores its 16 bytes directly.
-// Rows past T and dishes past D are clamped, always-in-bounds loads whose
-// values are replaced by zeros; the weights there are zero too. The plan
+// Rows past T and dishes past D are clamped, always-in-bounds loads: the
+// outputs of rows past T are never stored and the weights of dishes past D
+// are zero. The plan
 // routes gains outside the fast requantisation range (non-finite, or
 // |G * raw| >= 2^30 possible) to the generic kernel.
 #pragma once
@@ -41,18 +42,17 @@ namespace bf {
 
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-// Stores 16 bytes (w0..w3, little-endian) at dst with the widest accesses the
-// address alignment allows, keeping only bytes i < n.
+// Vector types without an alignment promise: gfx950 runs in unaligned
+// access mode, so a 16- or 8-byte store to any address is one instruction
+// (split by the memory system where it crosses a line).
+typedef v4i v4i_u __attribute__((aligned(1)));
+typedef int v2i __attribute__((ext_vector_type(2)));
+typedef v2i v2i_u __attribute__((aligned(1)));
+
+// Stores 16 bytes (w0..w3, little-endian) at dst, keeping only bytes i < n.
 __device__ __forceinline__ void store_row16(uint8_t *dst, uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, int n) {
-  const uintptr_t al = (uintptr_t)dst;
-  if (n >= 16 && (al & 15) == 0) {
-    *reinterpret_cast<uint4 *>(dst) = make_uint4(w0, w1, w2, w3);
-  } else if (n >= 16 && (al & 3) == 0) {
-    uint32_t *d = reinterpret_cast<uint32_t *>(dst);
-    d[0] = w0;
-    d[1] = w1;
-    d[2] = w2;
-    d[3] = w3;
+  if (n >= 16) {
+    *reinterpret_cast<v4i_u *>(dst) = v4i{(int)w0, (int)w1, (int)w2, (int)w3};
   } else {
     const uint32_t w[4] = {w0, w1, w2, w3};
 #pragma unroll
@@ -63,8 +63,8 @@ __device__ __forceinline__ void store_row16(uint8_t *dst, uint32_t w0, uint32_t 
 
 // Stores 8 bytes (w0, w1) at dst, keeping bytes i < n.
 __device__ __forceinline__ void store_row8(uint8_t *dst, uint32_t w0, uint32_t w1, int n) {
-  if (n >= 8 && ((uintptr_t)dst & 7) == 0) {
-    *reinterpret_cast<uint2 *>(dst) = make_uint2(w0, w1);
+  if (n >= 8) {
+    *reinterpret_cast<v2i_u *>(dst) = v2i{(int)w0, (int)w1};
   } else {
     const uint32_t w[2] = {w0, w1};
 #pragma unroll
@@ -121,11 +121,11 @@ struct TileJob {
   int32_t jlayout;   // 0 = [b][f][p][t], 1 = [b][f][t][p]
 };
 
-// Stores the 16 bytes (v.x .. v.w, little-endian) at dst. fast: dst is
-// 16-byte aligned and all 16 bytes are inside the row.
-__device__ __forceinline__ void store_piece(uint8_t *dst, const v4i v, int n, bool fast) {
-  if (fast)
-    *reinterpret_cast<v4i *>(dst) = v;
+// Stores the 16 bytes (v.x .. v.w, little-endian) at dst, keeping bytes
+// i < n. whole: every piece of the launch has n >= 16 (uniform).
+__device__ __forceinline__ void store_piece(uint8_t *dst, const v4i v, int n, bool whole) {
+  if (whole)
+    *reinterpret_cast<v4i_u *>(dst) = v;
   else
     store_row16(dst, (uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z, (uint32_t)v.w, n);
 }
@@ -145,12 +145,11 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
   constexpr int RP = RP0 < KP ? RP0 : KP;
   static_assert(KP % RP == 0, "ring slots must line up across tiles");
   constexpr int64_t step = kWaves;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t rounds = tile_end > tile_first ? (tile_end - tile_first + step - 1) / step : 0;
   int64_t tile = tile_first + wave;
-  // Whole-piece 16-byte stores: J rows start 16-byte aligned and every piece
-  // is complete (uniform).
-  const bool fast_store = ((uintptr_t)job.J & 15u) == 0 && (job.T & 15) == 0;
+  // Every 16-byte J piece is complete (uniform).
+  const bool fast_store = (job.T & 15) == 0;
 
   const int lane = threadIdx.x & 63;
   const int h = lane >> 5;          // k half (operands); row half (accumulators)
@@ -167,9 +166,9 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
   // step 2j + u for lane (h, pm, tm) is piece 2u + h of row tm (dishes
   // 16 (2j + u) + 8 h .. +7). Pieces are stored at slot (piece + row / 2) & 3
   // of their row so that the reads of 8 consecutive rows hit distinct banks.
-  // Loads are unconditional and in bounds (rows clamped to T - 1; with DT,
-  // dishes past D read the row's first bytes); out-of-range values are zeroed
-  // where they are consumed, never right after the load.
+  // Loads are unconditional and in bounds: rows past T re-read row T - 1
+  // (their outputs are computed but never stored) and, with DT, dishes past
+  // D read the row's first bytes (their weights are zero, so they add 0).
   const int ld_row = lane >> 2, ld_piece = lane & 3;
   const int64_t e_tstride = job.e_tstride;
   const uint8_t *Ef = job.E + ld_piece * 16;
@@ -178,8 +177,6 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
     const int64_t t = tl * kTileT + ld_row;
     return Ef + (t < job.T ? t : tlast) * e_tstride;
   };
-  // dishes of K step s for the operand lanes of half h
-  auto dish_ok = [&](int s) -> bool { return kDishesPerStep * s + 8 * h < job.D; };
   auto load_pair = [&](const uint8_t *p, int j) -> v4i {
     if (DT) return *reinterpret_cast<const v4i *>(p + (32 * j + 8 * ld_piece < job.D ? 64 * j : -16 * ld_piece));
     return *reinterpret_cast<const v4i *>(p + 64 * j);
@@ -208,24 +205,16 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
         acc[nt][1][j] = -c.y;
       }
     }
-    const bool row_ok = tile * kTileT + tm < job.T; // row of this lane's operand
     const uint8_t *pcur = row_ptr(tile);
     const uint8_t *pnext = row_ptr(tile + step);
 
-    // One K loop in straight-line code (a scheduling region per step): the
-    // ring of loads in flight never crosses a loop back-edge inside a tile,
-    // so the waits before each use stay exact. The loop is software
-    // pipelined: during step s the wave unpacks the operands of step s + 1
-    // (independent VALU work that overlaps the step's MFMAs; grouped after
-    // them rather than interleaved, which measures faster). MASK selects the
-    // variant that zeroes rows past T and dishes past D.
-    auto kloop = [&](auto mask_tag) {
-      constexpr bool MASK = decltype(mask_tag)::value;
-      auto operands = [&](v4i e, int s, v4i &v1, v4i &v2) {
-        if (MASK) {
-          const bool ok = row_ok && (!DT || dish_ok(s));
-          e = ok ? e : v4i{0, 0, 0, 0};
-        }
+    // One K loop in straight-line code (a scheduling region per step). The
+    // loop is software pipelined: during step s the wave unpacks the
+    // operands of step s + 1 (independent VALU work that overlaps the step's
+    // MFMAs; grouped after them rather than interleaved, which measures
+    // faster).
+    {
+      auto operands = [&](v4i e, v4i &v1, v4i &v2) {
         const v8i uo = unpack_e_pol(e, sel);
         v1 = op_v1(uo);
         v2 = op_v2(uo);
@@ -235,7 +224,7 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
       {
         xw[wr_slot] = ring[0];
         ring[0] = RP < KP ? load_pair(pcur, RP) : load_pair(pnext, RP - KP);
-        operands(xw[rd_slot[0]], 0, v1c, v2c);
+        operands(xw[rd_slot[0]], v1c, v2c);
       }
 #pragma unroll
       for (int s = 0; s < KS; ++s) {
@@ -250,7 +239,7 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
             const int j2 = j1 + RP;
             ring[q] = j2 >= KP ? load_pair(pnext, j2 - KP) : load_pair(pcur, j2);
           }
-          operands(xw[rd_slot[(s + 1) & 1]], s + 1, v1n, v2n);
+          operands(xw[rd_slot[(s + 1) & 1]], v1n, v2n);
         }
         v4i w[NT];
 #pragma unroll
@@ -264,11 +253,7 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
         v1c = v1n;
         v2c = v2n;
       }
-    };
-    if (DT || (tile + 1) * kTileT > job.T) // wave-uniform
-      kloop(std::true_type{});
-    else
-      kloop(std::false_type{});
+    }
 
     // Epilogue. Accumulator element j of lane (h, col) is output row
     // 8 (j / 4) + 4 h + j % 4 = (polarization j / 8, time 8 ((j / 4) & 1) +

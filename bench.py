@@ -80,10 +80,28 @@ def cpu_baseline(E_host, A, G, T, F, D, B, f_sample, J_gpu):
     }
 
 
-def timed(stream, steps, warmup, fn):
-    """Mean milliseconds of fn() over `steps` calls after `warmup` calls, with
-    HIP events on `stream` and a device synchronisation on both sides."""
+def settle_clock(fn, seconds):
+    """Runs fn() back to back for at least `seconds`: the GPU's engine clock
+    ramps up over tens of milliseconds after idle, so timing right after a
+    pause measures the ramp, not the kernel."""
     import torch
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(8):
+            fn()
+        torch.cuda.synchronize()
+        n += 8
+    return n
+
+
+def timed(stream, steps, warmup, fn, settle_s=0.0):
+    """Mean milliseconds of fn() over `steps` calls after `warmup` calls (and,
+    with settle_s, after the clock has settled), with HIP events on `stream`
+    and a device synchronisation on both sides."""
+    import torch
+    if settle_s > 0:
+        settle_clock(fn, settle_s)
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
@@ -129,9 +147,9 @@ def run_batched(bf, bo, torch, stream, steps, warmup, seed=7, C=64, D=512, B=96)
         for c in range(C):
             plan.form_beams_channel(c, Ts[c], dJp.data_ptr() + j_off[c], dE.data_ptr() + e_off[c], sptr)
 
-    ms_plain = timed(stream, steps, warmup, plain)
-    ms_batch = timed(stream, steps, warmup, lambda: batch.run(chunks, sptr))
-    ms_graph = timed(stream, steps, warmup, lambda: batch.run(chunks, sptr, graph=True))
+    ms_plain = timed(stream, steps, warmup, plain, settle_s=0.2)
+    ms_batch = timed(stream, steps, warmup, lambda: batch.run(chunks, sptr), settle_s=0.2)
+    ms_graph = timed(stream, steps, warmup, lambda: batch.run(chunks, sptr, graph=True), settle_s=0.2)
     same = bool(torch.equal(dJb, dJp))
     # the oracle on the two shortest chunks
     oracle_ok = True
@@ -273,6 +291,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--settle", type=float, default=0.3,
+                    help="seconds of untimed launches before the warmup steps, so the engine clock has ramped up")
     ap.add_argument("--ntimes", type=int, default=32768)
     ap.add_argument("--nfrequencies", type=int, default=32)
     ap.add_argument("--ndishes", type=int, default=512)
@@ -326,7 +346,8 @@ def main():
         log(f"kernel (tuned): {plan.describe()}")
 
     t0 = time.perf_counter()
-    ms = timed(stream, args.steps, args.warmup, lambda: plan.form_beams_device(dJ.data_ptr(), dE.data_ptr(), sptr))
+    ms = timed(stream, args.steps, args.warmup, lambda: plan.form_beams_device(dJ.data_ptr(), dE.data_ptr(), sptr),
+               settle_s=args.settle)
     wall = time.perf_counter() - t0  # warm-up and timed steps
 
     ops = workload_ops(T, F, D, B)
@@ -377,7 +398,8 @@ def main():
         "kernel": plan.describe(),
         "autotune": tune.strip().splitlines()[-1] if tune else None,
         "j_adler32": f"0x{checksum:08x}",
-        "wall_s_warmup_and_steps": round(wall, 4),
+        "wall_s_settle_warmup_and_steps": round(wall, 4),
+        "clock_settle_s": args.settle,
         "batched_vs_plain_launch": batched,
         "serving": serving,
         "cpu_baseline": cpu,

@@ -166,9 +166,10 @@ typedef struct bf_batch bf_batch;
 int bf_batch_create(const bf_plan *channel_plan, int max_chunks, bf_batch **batch);
 int bf_batch_destroy(bf_batch *batch);
 
-/* Work-item size in 16-sample tiles (rounded to a multiple of 8; 0 restores
- * the default). Smaller items balance better, larger ones restage weights
- * less often. */
+/* Fixed work-item size in 16-sample tiles (rounded to a multiple of 8).
+ * 0 restores the default, guided self-scheduling: items cut largest first,
+ * each about (remaining work) / (2 x grid) tiles, so the tail of the queue
+ * is fine-grained. */
 int bf_batch_set_item_tiles(bf_batch *batch, int item_tiles);
 
 /* Runs all chunks in one persistent launch with a device-side work queue, on
