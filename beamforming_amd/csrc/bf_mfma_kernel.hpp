@@ -208,50 +208,59 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
     const uint8_t *pcur = row_ptr(tile);
     const uint8_t *pnext = row_ptr(tile + step);
 
-    // One K loop in straight-line code (a scheduling region per step). The
-    // loop is software pipelined: during step s the wave unpacks the
-    // operands of step s + 1 (independent VALU work that overlaps the step's
-    // MFMAs; grouped after them rather than interleaved, which measures
-    // faster).
+    // One K loop in straight-line code, one scheduling region per step,
+    // software pipelined so that no step waits for LDS it has just asked
+    // for: during step s the wave reads the weight operands of step s + 1
+    // and the raw E operand of step s + 2 (staging a new pair into the
+    // transpose buffer when step s + 2 starts one), issues the MFMAs of
+    // step s, then unpacks the E operand of step s + 1 (VALU work grouped
+    // after the MFMAs, which co-executes better than interleaving it).
     {
       auto operands = [&](v4i e, v4i &v1, v4i &v2) {
         const v8i uo = unpack_e_pol(e, sel);
         v1 = op_v1(uo);
         v2 = op_v2(uo);
       };
-      // prologue: pair 0 through the transpose buffer, operands of step 0
-      v4i v1c, v2c;
+      // stages pair j into the transpose buffer and refills its ring slot
+      auto stage_pair = [&](int j) {
+        const int q = j % RP;
+        xw[wr_slot] = ring[q]; // the wave's LDS operations complete in order
+        const int j2 = j + RP;
+        ring[q] = j2 >= KP ? load_pair(pnext, j2 - KP) : load_pair(pcur, j2);
+      };
+      // prologue: pair 0 staged, raw E of steps 0 and 1, weights of step 0
+      v4i v1c, v2c, raw1, wc[NT];
       {
-        xw[wr_slot] = ring[0];
-        ring[0] = RP < KP ? load_pair(pcur, RP) : load_pair(pnext, RP - KP);
-        operands(xw[rd_slot[0]], v1c, v2c);
+        stage_pair(0);
+        const v4i raw0 = xw[rd_slot[0]];
+        raw1 = xw[rd_slot[1]];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) wc[nt] = Wl[nt * 64 + lane];
+        operands(raw0, v1c, v2c);
       }
 #pragma unroll
       for (int s = 0; s < KS; ++s) {
-        const bool more = s + 1 < KS;
-        v4i v1n = v1c, v2n = v2c;
-        if (more) {
-          if (s & 1) {
-            // next step starts pair j + 1: stage it, refill its ring slot
-            const int j1 = (s + 1) / 2;
-            const int q = j1 % RP;
-            xw[wr_slot] = ring[q]; // the wave's LDS operations complete in order
-            const int j2 = j1 + RP;
-            ring[q] = j2 >= KP ? load_pair(pnext, j2 - KP) : load_pair(pcur, j2);
-          }
-          operands(xw[rd_slot[(s + 1) & 1]], v1n, v2n);
-        }
-        v4i w[NT];
+        v4i wn[NT], raw2 = raw1, v1n = v1c, v2n = v2c;
+        if (s + 1 < KS) {
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) w[nt] = Wl[(s * NT + nt) * 64 + lane];
+          for (int nt = 0; nt < NT; ++nt) wn[nt] = Wl[((s + 1) * NT + nt) * 64 + lane];
+        }
+        if (s + 2 < KS) {
+          if (((s + 2) & 1) == 0) stage_pair((s + 2) / 2);
+          raw2 = xw[rd_slot[(s + 2) & 1]];
+        }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
-          acc[nt][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(v1c, w[nt], acc[nt][0], 0, 0, 0);
-          acc[nt][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(v2c, w[nt], acc[nt][1], 0, 0, 0);
+          acc[nt][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(v1c, wc[nt], acc[nt][0], 0, 0, 0);
+          acc[nt][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(v2c, wc[nt], acc[nt][1], 0, 0, 0);
         }
+        if (s + 1 < KS) operands(raw1, v1n, v2n);
         __builtin_amdgcn_sched_barrier(0);
         v1c = v1n;
         v2c = v2n;
+        raw1 = raw2;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) wc[nt] = wn[nt];
       }
     }
 
