@@ -19,7 +19,7 @@ import numpy as np
 __all__ = [
     "BF_J_BFPT", "BF_J_BFTP", "BeamformError", "Shape", "Plan", "lib", "library_path",
     "form_beams", "adler32", "adler32_device", "fill_uniform_device", "version", "Chunk", "Batch",
-    "Server", "ServerConfig", "Request", "Result", "ServerCounters",
+    "Server", "ServerConfig", "Request", "Result", "ServerCounters", "Ingest", "IngestConfig",
     "BF_ERR_INVALID_ARGUMENT", "BF_ERR_UNSUPPORTED", "BF_ERR_OVER_BUDGET", "BF_ERR_NOT_READY", "BF_ERR_CLOSED",
 ]
 
@@ -159,6 +159,21 @@ class ServerCounters(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class IngestConfig(ctypes.Structure):
+    """bf_ingest_config."""
+
+    _fields_ = [
+        ("nfrequencies", ctypes.c_int64),
+        ("ndishes", ctypes.c_int64),
+        ("npolarizations", ctypes.c_int64),
+        ("chunk_ntimes", ctypes.c_int64),
+        ("group_channels", ctypes.c_int32),
+        ("ring_slots", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("reserved2", ctypes.c_int32),
+    ]
+
+
 def _load():
     if not os.path.exists(library_path):
         raise ImportError(f"{library_path} is not built; run `make -C {_HERE}` (hipcc, gfx950)")
@@ -202,6 +217,12 @@ def _load():
         "bf_server_release": (ctypes.c_int, [vp, ctypes.c_uint64]),
         "bf_server_stats": (ctypes.c_int, [vp, ctypes.POINTER(ServerCounters)]),
         "bf_server_clock": (ctypes.c_double, [vp]),
+        "bf_ingest_create": (ctypes.c_int, [ctypes.POINTER(IngestConfig), ctypes.POINTER(vp)]),
+        "bf_ingest_destroy": (ctypes.c_int, [vp]),
+        "bf_ingest_push": (ctypes.c_int, [vp, u8p, ctypes.c_int64, ctypes.c_int]),
+        "bf_ingest_pop": (ctypes.c_int, [vp, vp, ctypes.POINTER(Chunk), ctypes.POINTER(ctypes.c_uint64)]),
+        "bf_ingest_release": (ctypes.c_int, [vp, ctypes.c_uint64, vp]),
+        "bf_ingest_pending": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -472,3 +493,56 @@ class Server:
 
     def clock(self):
         return lib.bf_server_clock(self._h)
+
+
+class Ingest:
+    """E ingest: packets E[t][F][D][P] into per-channel-group device chunks."""
+
+    def __init__(self, nfrequencies, ndishes, chunk_ntimes, group_channels, ring_slots=4, npolarizations=2):
+        cfg = IngestConfig(nfrequencies, ndishes, npolarizations, chunk_ntimes, group_channels, ring_slots, 0, 0)
+        self._h = ctypes.c_void_p()
+        _check(lib.bf_ingest_create(ctypes.byref(cfg), ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib.bf_ingest_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def push(self, packet_ptr, ntimes, on_host):
+        """Appends a packet; returns False (nothing copied) when no ring slot is free."""
+        st = lib.bf_ingest_push(self._h, ctypes.c_void_p(packet_ptr), ntimes, int(bool(on_host)))
+        if st == BF_ERR_NOT_READY:
+            return False
+        _check(st)
+        return True
+
+    def pop(self, stream=0):
+        """Next complete chunk as (Chunk, slot_id), or None."""
+        c = Chunk()
+        sid = ctypes.c_uint64()
+        st = lib.bf_ingest_pop(self._h, ctypes.c_void_p(stream), ctypes.byref(c), ctypes.byref(sid))
+        if st == BF_ERR_NOT_READY:
+            return None
+        _check(st)
+        return c, sid.value
+
+    def release(self, slot_id, stream=0):
+        _check(lib.bf_ingest_release(self._h, slot_id, ctypes.c_void_p(stream)))
+
+    def pending(self):
+        n = ctypes.c_int64()
+        r = ctypes.c_int()
+        _check(lib.bf_ingest_pending(self._h, ctypes.byref(n), ctypes.byref(r)))
+        return n.value, r.value

@@ -289,6 +289,49 @@ int bf_server_stats(bf_server *server, bf_server_counters *counters);
 /* Seconds since the server was created (the clock of bf_result). */
 double bf_server_clock(const bf_server *server);
 
+/* ---------------------------------------------------------------------------
+ * E ingest: packets of ntimes time samples for all F channels, in the
+ * reference layout E[t][f][d][p] (Elinear, arraysizes.hxx:79-89), are cut
+ * into chunks of chunk_ntimes samples for each group of group_channels
+ * consecutive channels and written into per-group device ring buffers
+ * ([chunk_ntimes][G][D][P] per slot), by strided 2-D copies on the ingest's
+ * own stream. A popped chunk is a bf_chunk (nfreq = G, J left NULL) for
+ * bf_batch_run or the server. One thread drives an ingest object.
+ * ------------------------------------------------------------------------- */
+
+typedef struct bf_ingest bf_ingest;
+
+typedef struct bf_ingest_config {
+  int64_t nfrequencies;   /* F: channels in every packet */
+  int64_t ndishes;        /* D: multiple of 8 */
+  int64_t npolarizations; /* must be 2 */
+  int64_t chunk_ntimes;   /* samples per output chunk */
+  int32_t group_channels; /* G: channels per output chunk; divides F */
+  int32_t ring_slots;     /* chunks buffered per group, >= 2 */
+  int32_t reserved;       /* must be 0 */
+  int32_t reserved2;      /* must be 0 */
+} bf_ingest_config;
+
+int bf_ingest_create(const bf_ingest_config *config, bf_ingest **ingest);
+int bf_ingest_destroy(bf_ingest *ingest);
+
+/* Appends a packet of ntimes samples (E[ntimes][F][D][P]; host memory if
+ * packet_on_host, best pinned, else device memory). Packets continue each
+ * other in time. Returns BF_ERR_NOT_READY, copying nothing, when a chunk it
+ * would start has no free ring slot (the consumer has not released one). */
+int bf_ingest_push(bf_ingest *ingest, const uint8_t *packet, int64_t ntimes, int packet_on_host);
+
+/* Next complete chunk, in completion order (for every group in turn), or
+ * BF_ERR_NOT_READY. `stream` is made to wait for the chunk's copies. */
+int bf_ingest_pop(bf_ingest *ingest, void *stream, bf_chunk *chunk, uint64_t *slot_id);
+
+/* Returns a popped chunk's slot to the ring once the work enqueued on
+ * `stream` so far (the consumer of the chunk) has finished. */
+int bf_ingest_release(bf_ingest *ingest, uint64_t slot_id, void *stream);
+
+/* Samples in the chunks being filled and complete chunks not yet popped. */
+int bf_ingest_pending(const bf_ingest *ingest, int64_t *filled_samples, int *ready_chunks);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,95 @@
+"""GPU tests of E ingest: packets E[t][F][D][P] (reference Elinear) become
+per-channel-group chunks [T_chunk][G][D][P] in device ring buffers. The
+chunks are checked byte for byte against the packets' slices, and beams
+formed from them through the batch path against the oracle."""
+import numpy as np
+import pytest
+import torch
+
+import beamforming_amd as bf
+from oracle import bf_oracle as bo
+
+pytestmark = pytest.mark.gpu
+
+
+def stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def device_bytes(ptr, n):
+    """Copies n bytes of device memory at ptr to a new host array (HIP runtime)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    host = np.empty(n, dtype=np.uint8)
+    assert hip.hipMemcpy(host.ctypes.data, ctypes.c_void_p(ptr), n, 2) == 0  # hipMemcpyDeviceToHost
+    return host
+
+
+@pytest.mark.parametrize("on_host", [True, False])
+def test_ingest_chunks_are_packet_slices(on_host):
+    F, D, G, TC = 8, 64, 4, 100
+    total = 3 * TC + 57
+    E = bo.fill_uniform(total * F * D * 2, 71).reshape(total, F, D * 2)
+    sizes = [1, 17, 100, 64, 99, 3, 53]  # ragged packets crossing chunk boundaries
+    sizes.append(total - sum(sizes))
+    assert sizes[-1] > 0
+    dE = torch.from_numpy(E.reshape(-1)).cuda()
+    got = []
+    with bf.Ingest(F, D, TC, G, ring_slots=4) as ing:
+        t = 0
+        for n in sizes:
+            if on_host:
+                pkt = np.ascontiguousarray(E[t:t + n]).reshape(-1)
+                assert ing.push(pkt.ctypes.data, n, True)
+                torch.cuda.synchronize()
+            else:
+                assert ing.push(dE.data_ptr() + t * F * D * 2, n, False)
+            t += n
+            while True:
+                r = ing.pop(stream())
+                if r is None:
+                    break
+                c, sid = r
+                torch.cuda.synchronize()
+                got.append((c.channel, c.nfreq, c.ntimes, device_bytes(c.E, TC * G * D * 2)))
+                ing.release(sid, stream())
+        filled, ready = ing.pending()
+        assert filled == total % TC and ready == 0
+    assert len(got) == (total // TC) * (F // G)
+    for i, (ch, nf, nt, data) in enumerate(got):
+        k = i // (F // G)  # chunk index in time
+        assert nf == G and nt == TC and ch == (i % (F // G)) * G
+        want = E[k * TC:(k + 1) * TC, ch:ch + G].reshape(-1)
+        assert np.array_equal(data, want), f"chunk {i}"
+
+
+def test_ingest_backpressure_and_beams():
+    F, D, B, G, TC = 8, 512, 96, 4, 256
+    A = bo.fill_uniform(F * B * D, 72)
+    Gn = bo.setup_gain(F * B, D) * np.float32(8)
+    E = bo.fill_uniform(4 * TC * F * D * 2, 73).reshape(4 * TC, F, D * 2)
+    dE = torch.from_numpy(E.reshape(-1)).cuda()
+    with bf.Plan(bf.Shape.make(1, F, D, B), A, Gn) as plan, bf.Batch(plan, 8) as batch, \
+            bf.Ingest(F, D, TC, G, ring_slots=2) as ing:
+        # two chunks fill both slots of every group; the third needs a released slot
+        assert ing.push(dE.data_ptr(), 2 * TC, False)
+        popped = [ing.pop(stream()) for _ in range(2 * (F // G))]
+        assert all(p is not None for p in popped) and ing.pop(stream()) is None
+        assert not ing.push(dE.data_ptr() + 2 * TC * F * D * 2, TC, False)  # no free slot: refused
+        assert ing.pending() == (0, 0)
+        # beams from the popped chunks, then release their slots after that work
+        Js = [torch.zeros(B * G * 2 * TC, dtype=torch.uint8, device="cuda") for _ in popped]
+        chunks = [(c.E, J.data_ptr(), c.ntimes, c.channel, c.nfreq) for (c, _), J in zip(popped, Js)]
+        batch.run(chunks, stream())
+        for _, sid in popped:
+            ing.release(sid, stream())
+        torch.cuda.synchronize()
+        assert ing.push(dE.data_ptr() + 2 * TC * F * D * 2, TC, False)  # slots are free again
+        torch.cuda.synchronize()
+    for i, ((c, _), J) in enumerate(zip(popped, Js)):
+        k = i // (F // G)
+        ch = c.channel
+        Ek = np.ascontiguousarray(E[k * TC:(k + 1) * TC, ch:ch + G]).reshape(-1)
+        want = bo.form_beams(Ek, A.reshape(F, -1)[ch:ch + G], Gn.reshape(F, -1)[ch:ch + G], TC, G, D, B)
+        assert np.array_equal(J.cpu().numpy(), want), f"chunk {i}"

@@ -92,3 +92,19 @@ def test_no_device_is_reported_not_fatal():
     with pytest.raises(bf.BeamformError) as ei:
         bf.Plan(shape, A, G)
     assert ei.value.status in (3, 8)  # device error / no device
+
+
+@pytest.mark.parametrize("bad", [
+    dict(nfrequencies=8, ndishes=64, chunk_ntimes=16, group_channels=3, ring_slots=2),   # G does not divide F
+    dict(nfrequencies=8, ndishes=64, chunk_ntimes=16, group_channels=4, ring_slots=1),   # one slot
+    dict(nfrequencies=8, ndishes=64, chunk_ntimes=0, group_channels=4, ring_slots=2),    # empty chunks
+    dict(nfrequencies=8, ndishes=64, chunk_ntimes=16, group_channels=4, ring_slots=2, npolarizations=1),
+])
+def test_ingest_config_rejected_before_any_device_call(bad):
+    args = dict(bad)
+    p = args.pop("npolarizations", 2)
+    cfg = bf.IngestConfig(args["nfrequencies"], args["ndishes"], p, args["chunk_ntimes"], args["group_channels"],
+                          args["ring_slots"], 0, 0)
+    h = ctypes.c_void_p()
+    assert bf.lib.bf_ingest_create(ctypes.byref(cfg), ctypes.byref(h)) == bf.BF_ERR_INVALID_ARGUMENT
+    assert not h.value
