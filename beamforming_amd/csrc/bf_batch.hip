@@ -6,7 +6,6 @@
 #include <algorithm>
 #include <cstdio>
 #include <functional>
-#include <queue>
 #include <string>
 #include <vector>
 
@@ -25,14 +24,15 @@ struct bf_batch {
   int item_tiles = 0; // 0 = automatic
   int last_item_tiles = 0;
   int last_nitems = 0;
+  int last_segments = 0;
   int grid = 0;
-  BatchChunk *d_chunks = nullptr;
-  BatchItem *d_items = nullptr;
-  int *d_nitems = nullptr;
-  unsigned *d_head = nullptr;
-  BatchChunk *h_chunks = nullptr; // pinned staging
-  BatchItem *h_items = nullptr;   // pinned staging
-  int *h_nitems = nullptr;        // pinned staging
+  // One descriptor block, uploaded with a single copy per launch:
+  // [work item count, queue head (0) | pad to 16][chunks: max_chunks]
+  // [work item starts: item_capacity + 1, padded to 16][segments:
+  // item_capacity]. The copy also resets the queue head.
+  uint8_t *d_desc = nullptr;
+  uint8_t *h_desc = nullptr; // pinned staging
+  size_t off_chunks = 16, off_work = 0, off_items = 0, desc_bytes = 0;
   hipEvent_t upload_done = nullptr;
   bool upload_pending = false;
   hipStream_t capture_stream = nullptr;
@@ -67,13 +67,20 @@ int ensure_item_capacity(bf_batch *b, int n) {
     BF_HIP(hipEventSynchronize(b->upload_done));
     b->upload_pending = false;
   }
-  if (b->d_items) (void)hipFree(b->d_items);
-  if (b->h_items) (void)hipHostFree(b->h_items);
-  b->d_items = nullptr;
-  b->h_items = nullptr;
+  // the previous block may still be read by a launch in flight
+  BF_HIP(hipDeviceSynchronize());
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->h_desc) (void)hipHostFree(b->h_desc);
+  b->d_desc = nullptr;
+  b->h_desc = nullptr;
   b->item_capacity = 0;
-  BF_HIP(hipMalloc(&b->d_items, (size_t)cap * sizeof(BatchItem)));
-  BF_HIP(hipHostMalloc(&b->h_items, (size_t)cap * sizeof(BatchItem), hipHostMallocDefault));
+  auto up16 = [](size_t x) { return (x + 15) / 16 * 16; };
+  b->off_chunks = 16;
+  b->off_work = b->off_chunks + up16((size_t)b->max_chunks * sizeof(BatchChunk));
+  b->off_items = b->off_work + up16((size_t)(cap + 1) * sizeof(int));
+  b->desc_bytes = b->off_items + (size_t)cap * sizeof(BatchItem);
+  BF_HIP(hipMalloc(&b->d_desc, b->desc_bytes));
+  BF_HIP(hipHostMalloc(&b->h_desc, b->desc_bytes, hipHostMallocDefault));
   b->item_capacity = cap;
   free_graph(b); // the captured launch refers to the old item buffer
   return BF_OK;
@@ -93,37 +100,35 @@ void fixed_items(const bf_chunk *chunks, int n, int nbg, int it, std::vector<Bat
   }
 }
 
-// Guided self-scheduling: items are cut, largest first, from the longest
-// remaining piece of any (chunk, frequency, beam group), each about
-// remaining / (2 grid) tiles (a multiple of the 8 waves, between 32 and 256;
-// every item start costs one E pipeline fill, so the tail is not cut finer
-// than 4 rounds). Workgroups take them in this order from the queue, so the
-// big items are spread over the grid first and the shrinking tail fills in
-// the imbalance.
-void guided_items(const bf_chunk *chunks, int n, int nbg, int grid, int64_t total, std::vector<BatchItem> &items) {
-  struct Piece {
-    int64_t begin, end;
-    int chunk, f, bg;
-    bool operator<(const Piece &o) const { return end - begin < o.end - o.begin; }
-  };
-  std::priority_queue<Piece> heap;
+// Balanced work items: the (chunk, frequency, beam group) tile ranges are
+// laid end to end and cut into one share of about total / grid tiles (a
+// multiple of the 8 waves) per workgroup; a share that crosses a chunk
+// boundary becomes several segments of one item. Every workgroup then takes
+// one item from the queue, the E pipeline restarts only at segment starts,
+// and the weights are restaged only where a share crosses into another
+// channel.
+void balanced_items(const bf_chunk *chunks, int n, int nbg, int grid, int64_t total, std::vector<BatchItem> &segs,
+                    std::vector<int> &work) {
+  const int64_t share = std::max<int64_t>(kWaves, round_up_waves((total + grid - 1) / grid));
+  int64_t fill = 0;
+  work.push_back(0);
   for (int i = 0; i < n; ++i) {
     const int64_t nt = (chunks[i].ntimes + kTileT - 1) / kTileT;
     const int nf = chunk_nfreq(chunks[i]);
     for (int f = 0; f < nf; ++f)
-      for (int bg = 0; bg < nbg; ++bg) heap.push(Piece{0, nt, i, f, bg});
+      for (int bg = 0; bg < nbg; ++bg)
+        for (int64_t t = 0; t < nt;) {
+          const int64_t take = std::min(nt - t, share - fill);
+          segs.push_back(BatchItem{i, (uint16_t)f, (uint16_t)bg, (int32_t)t, (int32_t)(t + take)});
+          t += take;
+          fill += take;
+          if (fill == share) {
+            work.push_back((int)segs.size());
+            fill = 0;
+          }
+        }
   }
-  int64_t remaining = total;
-  while (!heap.empty()) {
-    Piece p = heap.top();
-    heap.pop();
-    const int64_t want = std::min<int64_t>(256, std::max<int64_t>(4 * kWaves, round_up_waves(remaining / (2 * (int64_t)grid))));
-    const int64_t size = std::min(want, p.end - p.begin);
-    items.push_back(BatchItem{p.chunk, (uint16_t)p.f, (uint16_t)p.bg, (int32_t)p.begin, (int32_t)(p.begin + size)});
-    remaining -= size;
-    p.begin += size;
-    if (p.begin < p.end) heap.push(p);
-  }
+  if (fill > 0) work.push_back((int)segs.size());
 }
 
 // Validates the chunk list, builds the work items (longest first) into the
@@ -147,44 +152,52 @@ int upload(bf_batch *b, const bf_chunk *chunks, int n, hipStream_t st) {
   }
   for (int i = 0; i < n; ++i)
     if ((chunks[i].ntimes + kTileT - 1) / kTileT > (1ll << 30)) return fail(BF_ERR_UNSUPPORTED, "chunk too long");
-  std::vector<BatchItem> items;
+  std::vector<BatchItem> items; // segments
+  std::vector<int> work;        // work item starts
+  int it = 0;
   if (b->item_tiles > 0) {
     fixed_items(chunks, n, c.nbg, b->item_tiles, items);
-    // longest first
+    // longest first, one segment per work item
     std::stable_sort(items.begin(), items.end(), [](const BatchItem &x, const BatchItem &y) {
       return x.tile_end - x.tile_begin > y.tile_end - y.tile_begin;
     });
+    for (int i = 0; i <= (int)items.size(); ++i) work.push_back(i);
+    it = b->item_tiles;
   } else {
-    guided_items(chunks, n, c.nbg, b->grid, total_tiles, items);
+    balanced_items(chunks, n, c.nbg, b->grid, total_tiles, items, work);
+    it = (int)std::max<int64_t>(kWaves, round_up_waves((total_tiles + b->grid - 1) / b->grid));
   }
-  const int it = items.empty() ? 0 : items.front().tile_end - items.front().tile_begin;
+  const int nwork = (int)work.size() - 1;
   if (int rc = ensure_item_capacity(b, std::max<int>(1, (int)items.size()))) return rc;
   if (b->upload_pending) { // the staging buffers may still be read by the previous copy
     BF_HIP(hipEventSynchronize(b->upload_done));
     b->upload_pending = false;
   }
+  BatchChunk *hc = reinterpret_cast<BatchChunk *>(b->h_desc + b->off_chunks);
   for (int i = 0; i < n; ++i)
-    b->h_chunks[i] = BatchChunk{chunks[i].E, chunks[i].J, chunks[i].ntimes, chunks[i].channel, chunk_nfreq(chunks[i])};
-  std::copy(items.begin(), items.end(), b->h_items);
-  *b->h_nitems = (int)items.size();
-  if (n > 0) BF_HIP(hipMemcpyAsync(b->d_chunks, b->h_chunks, (size_t)n * sizeof(BatchChunk), hipMemcpyHostToDevice, st));
-  if (!items.empty())
-    BF_HIP(hipMemcpyAsync(b->d_items, b->h_items, items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, st));
-  BF_HIP(hipMemcpyAsync(b->d_nitems, b->h_nitems, sizeof(int), hipMemcpyHostToDevice, st));
+    hc[i] = BatchChunk{chunks[i].E, chunks[i].J, chunks[i].ntimes, chunks[i].channel, chunk_nfreq(chunks[i])};
+  std::copy(items.begin(), items.end(), reinterpret_cast<BatchItem *>(b->h_desc + b->off_items));
+  std::copy(work.begin(), work.end(), reinterpret_cast<int *>(b->h_desc + b->off_work));
+  reinterpret_cast<int *>(b->h_desc)[0] = nwork;
+  reinterpret_cast<unsigned *>(b->h_desc)[1] = 0; // queue head
+  BF_HIP(hipMemcpyAsync(b->d_desc, b->h_desc, b->off_items + items.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
+                        st));
   BF_HIP(hipEventRecord(b->upload_done, st));
   b->upload_pending = true;
   b->last_item_tiles = it;
-  b->last_nitems = (int)items.size();
+  b->last_nitems = nwork;
+  b->last_segments = (int)items.size();
   return BF_OK;
 }
 
 BatchParams batch_params(const bf_batch *b) {
   const bf_plan *p = b->plan;
   BatchParams a;
-  a.chunks = b->d_chunks;
-  a.items = b->d_items;
-  a.nitems = b->d_nitems;
-  a.head = b->d_head;
+  a.chunks = reinterpret_cast<const BatchChunk *>(b->d_desc + b->off_chunks);
+  a.items = reinterpret_cast<const BatchItem *>(b->d_desc + b->off_items);
+  a.work_start = reinterpret_cast<const int *>(b->d_desc + b->off_work);
+  a.nitems = reinterpret_cast<const int *>(b->d_desc);
+  a.head = reinterpret_cast<unsigned *>(b->d_desc) + 1;
   a.W = p->W;
   a.corr = p->corr;
   a.G = p->Gp;
@@ -195,11 +208,10 @@ BatchParams batch_params(const bf_batch *b) {
   return a;
 }
 
-// The launch itself: queue reset, then the persistent kernel.
+// The launch itself: the persistent kernel (the descriptor copy before it
+// has reset the queue head).
 int enqueue_launch(bf_batch *b, hipStream_t st) {
   const MfmaConfig &c = b->plan->cfg;
-  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, st, b->d_head);
-  BF_HIP(hipGetLastError());
   BatchParams a = batch_params(b);
   void *args[] = {&a};
   BF_HIP(hipLaunchKernel(reinterpret_cast<const void *>(c.batch_kernel), dim3(b->grid), dim3(kThreads), args,
@@ -282,14 +294,6 @@ int bf_batch_create(const bf_plan *p, int max_chunks, bf_batch **out) {
       per_cu < 1)
     per_cu = 1;
   b->grid = p->num_cus * per_cu;
-  if ((e = hipMalloc(&b->d_chunks, (size_t)max_chunks * sizeof(BatchChunk))) != hipSuccess)
-    return bail(hip_fail(e, "hipMalloc chunks"));
-  if ((e = hipMalloc(&b->d_nitems, sizeof(int))) != hipSuccess) return bail(hip_fail(e, "hipMalloc nitems"));
-  if ((e = hipMalloc(&b->d_head, sizeof(unsigned))) != hipSuccess) return bail(hip_fail(e, "hipMalloc head"));
-  if ((e = hipHostMalloc(&b->h_chunks, (size_t)max_chunks * sizeof(BatchChunk), hipHostMallocDefault)) != hipSuccess)
-    return bail(hip_fail(e, "hipHostMalloc chunks"));
-  if ((e = hipHostMalloc(&b->h_nitems, sizeof(int), hipHostMallocDefault)) != hipSuccess)
-    return bail(hip_fail(e, "hipHostMalloc nitems"));
   if ((e = hipEventCreateWithFlags(&b->upload_done, hipEventDisableTiming)) != hipSuccess)
     return bail(hip_fail(e, "hipEventCreate"));
   if ((e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking)) != hipSuccess)
@@ -305,13 +309,8 @@ int bf_batch_destroy(bf_batch *b) {
   free_graph(b);
   if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
   if (b->upload_done) (void)hipEventDestroy(b->upload_done);
-  if (b->d_chunks) (void)hipFree(b->d_chunks);
-  if (b->d_items) (void)hipFree(b->d_items);
-  if (b->d_nitems) (void)hipFree(b->d_nitems);
-  if (b->d_head) (void)hipFree(b->d_head);
-  if (b->h_chunks) (void)hipHostFree(b->h_chunks);
-  if (b->h_items) (void)hipHostFree(b->h_items);
-  if (b->h_nitems) (void)hipHostFree(b->h_nitems);
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->h_desc) (void)hipHostFree(b->h_desc);
   delete b;
   return BF_OK;
 }
@@ -357,9 +356,10 @@ int bf_batch_describe(const bf_batch *b, char *buf, size_t len) {
   const MfmaConfig &c = b->plan->cfg;
   std::snprintf(buf, len,
                 "persistent mfma_i32_32x32x32_i8 batch: nt=%d ks=%d nbg=%d grid=%d lds=%zu item_tiles=%d%s items=%d "
+                "segments=%d "
                 "graph=%s",
-                c.nt, c.ks, c.nbg, b->grid, c.lds_bytes, b->last_item_tiles, b->item_tiles ? "" : " (largest, guided)",
-                b->last_nitems, b->exec ? "captured" : "none");
+                c.nt, c.ks, c.nbg, b->grid, c.lds_bytes, b->last_item_tiles, b->item_tiles ? "" : " (balanced)",
+                b->last_nitems, b->last_segments, b->exec ? "captured" : "none");
   return BF_OK;
 }
 

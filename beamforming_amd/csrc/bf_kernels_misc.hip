@@ -143,10 +143,6 @@ __global__ void adler32_partial_kernel(const uint8_t *__restrict__ data, uint64_
   }
 }
 
-__global__ void batch_reset_kernel(unsigned *head) {
-  if (threadIdx.x == 0) *head = 0u;
-}
-
 __global__ void zero_u64_kernel(unsigned long long *p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;

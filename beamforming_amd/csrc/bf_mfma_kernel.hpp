@@ -42,6 +42,16 @@ namespace bf {
 
 typedef int v16i __attribute__((ext_vector_type(16)));
 
+// Global-memory views. E and J are always device memory, but pointers read
+// from a chunk table are generic to the compiler; generic (flat) accesses
+// also count as LDS operations, and a pending one makes every wait a full
+// drain of both counters. Every E load and J store goes through these.
+#define BF_GLOBAL __attribute__((address_space(1)))
+template <class T> __device__ __forceinline__ const BF_GLOBAL T *gload_ptr(const void *p) {
+  return (const BF_GLOBAL T *)p;
+}
+template <class T> __device__ __forceinline__ BF_GLOBAL T *gstore_ptr(void *p) { return (BF_GLOBAL T *)p; }
+
 // Vector types without an alignment promise: gfx950 runs in unaligned
 // access mode, so a 16- or 8-byte store to any address is one instruction
 // (split by the memory system where it crosses a line).
@@ -52,24 +62,26 @@ typedef v2i v2i_u __attribute__((aligned(1)));
 // Stores 16 bytes (w0..w3, little-endian) at dst, keeping only bytes i < n.
 __device__ __forceinline__ void store_row16(uint8_t *dst, uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, int n) {
   if (n >= 16) {
-    *reinterpret_cast<v4i_u *>(dst) = v4i{(int)w0, (int)w1, (int)w2, (int)w3};
+    *gstore_ptr<v4i_u>(dst) = v4i{(int)w0, (int)w1, (int)w2, (int)w3};
   } else {
     const uint32_t w[4] = {w0, w1, w2, w3};
+    BF_GLOBAL uint8_t *d = gstore_ptr<uint8_t>(dst);
 #pragma unroll
     for (int i = 0; i < 16; ++i)
-      if (i < n) dst[i] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+      if (i < n) d[i] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
   }
 }
 
 // Stores 8 bytes (w0, w1) at dst, keeping bytes i < n.
 __device__ __forceinline__ void store_row8(uint8_t *dst, uint32_t w0, uint32_t w1, int n) {
   if (n >= 8) {
-    *reinterpret_cast<v2i_u *>(dst) = v2i{(int)w0, (int)w1};
+    *gstore_ptr<v2i_u>(dst) = v2i{(int)w0, (int)w1};
   } else {
     const uint32_t w[2] = {w0, w1};
+    BF_GLOBAL uint8_t *d = gstore_ptr<uint8_t>(dst);
 #pragma unroll
     for (int i = 0; i < 8; ++i)
-      if (i < n) dst[i] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+      if (i < n) d[i] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
   }
 }
 
@@ -125,7 +137,7 @@ struct TileJob {
 // i < n. whole: every piece of the launch has n >= 16 (uniform).
 __device__ __forceinline__ void store_piece(uint8_t *dst, const v4i v, int n, bool whole) {
   if (whole)
-    *reinterpret_cast<v4i_u *>(dst) = v;
+    *gstore_ptr<v4i_u>(dst) = v;
   else
     store_row16(dst, (uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z, (uint32_t)v.w, n);
 }
@@ -178,14 +190,19 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
     return Ef + (t < job.T ? t : tlast) * e_tstride;
   };
   auto load_pair = [&](const uint8_t *p, int j) -> v4i {
-    if (DT) return *reinterpret_cast<const v4i *>(p + (32 * j + 8 * ld_piece < job.D ? 64 * j : -16 * ld_piece));
-    return *reinterpret_cast<const v4i *>(p + 64 * j);
+    if (DT) return *gload_ptr<v4i>(p + (32 * j + 8 * ld_piece < job.D ? 64 * j : -16 * ld_piece));
+    return *gload_ptr<v4i>(p + 64 * j);
   };
   const int wr_slot = ld_row * 4 + ((ld_piece + (ld_row >> 1)) & 3);
   int rd_slot[2];
 #pragma unroll
   for (int u = 0; u < 2; ++u) rd_slot[u] = tm * 4 + ((2 * u + h + (tm >> 1)) & 3);
 
+  // Nothing issued before this call is waited for inside it: completing it
+  // here gives the compiler an empty count of memory operations in flight at
+  // the loop below, so its waits on the E ring stay exact when the caller
+  // itself loops (the batch kernel runs one call per segment).
+  __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0)
   v4i ring[RP];
   {
     const uint8_t *p0 = row_ptr(tile);
@@ -398,10 +415,11 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
 
 // ---------------------------------------------------------------------------
 // Batched form_beams: independent chunks (nf consecutive channels each, own
-// T) in one persistent launch. The host builds a list of work items (chunk,
-// frequency of the chunk, beam group, tile range), longest first; workgroups
-// take items from a device-side queue (an atomic head counter) until it is
-// empty, restaging the weights in LDS only when the (channel, beam group)
+// T) in one persistent launch. The host cuts the work into segments (chunk,
+// frequency of the chunk, beam group, tile range) and groups consecutive
+// segments into work items; workgroups take items from a device-side queue
+// (an atomic head counter) until it is empty and run the item's segments in
+// order, restaging the weights in LDS only when the (channel, beam group)
 // changes.
 struct BatchChunk {
   const uint8_t *E; // [T][nf][D][2]
@@ -417,9 +435,10 @@ struct BatchItem {
 };
 struct BatchParams {
   const BatchChunk *chunks;
-  const BatchItem *items;
-  const int *nitems;  // device: number of valid items
-  unsigned *head;     // device: queue head, zeroed before each launch
+  const BatchItem *items;   // segments
+  const int *work_start;    // item w = segments work_start[w] .. work_start[w + 1] - 1
+  const int *nitems;        // device: number of work items
+  unsigned *head;     // device: queue head, zeroed by the descriptor copy before each launch
   const v4i *W;       // channel table [C][KS][NTP][64]
   const int *corr;    // [C][BP][2]
   const float *G;     // [C][BP]
@@ -442,32 +461,34 @@ __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a
     __syncthreads();
     const int it = __builtin_amdgcn_readfirstlane(s_item);
     if (it >= nitems) break; // uniform: every thread read the same s_item
-    const BatchItem item = a.items[it];
-    const BatchChunk ch = a.chunks[item.chunk];
-    const int channel = ch.channel + item.f;
-    if (channel != loaded_channel || item.bg != loaded_bg) {
-      stage_weights<NT, KS>(a.W, a.corr, a.G, a.NTP, channel, item.bg, Wl, Cl, Gl);
-      loaded_channel = channel;
-      loaded_bg = item.bg;
+    const int seg_end = a.work_start[it + 1];
+    for (int sg = a.work_start[it]; sg < seg_end; ++sg) {
+      const BatchItem item = a.items[sg];
+      const BatchChunk ch = a.chunks[item.chunk];
+      const int channel = ch.channel + item.f;
+      if (channel != loaded_channel || item.bg != loaded_bg) {
+        stage_weights<NT, KS>(a.W, a.corr, a.G, a.NTP, channel, item.bg, Wl, Cl, Gl);
+        loaded_channel = channel;
+        loaded_bg = item.bg;
+      }
+      __syncthreads();
+      TileJob job;
+      job.E = ch.E + (int64_t)item.f * a.D * 2;
+      job.e_tstride = (int64_t)ch.nf * a.D * 2;
+      job.J = ch.J;
+      job.T = ch.T;
+      job.F = ch.nf;
+      job.f = item.f;
+      job.D = a.D;
+      job.B = a.B;
+      job.bg = item.bg;
+      job.jlayout = a.jlayout;
+      run_tiles<NT, KS, DT>(job, item.tile_begin, item.tile_end, Wl, Cl, Gl, Xl + wave * 64, Sl);
+      __syncthreads(); // the LDS weights and staging are reused by the next segment
     }
-    __syncthreads();
-    TileJob job;
-    job.E = ch.E + (int64_t)item.f * a.D * 2;
-    job.e_tstride = (int64_t)ch.nf * a.D * 2;
-    job.J = ch.J;
-    job.T = ch.T;
-    job.F = ch.nf;
-    job.f = item.f;
-    job.D = a.D;
-    job.B = a.B;
-    job.bg = item.bg;
-    job.jlayout = a.jlayout;
-    run_tiles<NT, KS, DT>(job, item.tile_begin, item.tile_end, Wl, Cl, Gl, Xl + wave * 64, Sl);
-    __syncthreads(); // LDS and s_item are reused by the next item
+    __syncthreads(); // s_item is rewritten by the next pop
   }
 }
-
-__global__ void batch_reset_kernel(unsigned *head);
 
 // Kernel tables: one instance per (NT, KS, DT), registered by the
 // bf_mfma_ks*.hip translation units. DT = true handles ndishes that are not

@@ -166,10 +166,11 @@ typedef struct bf_batch bf_batch;
 int bf_batch_create(const bf_plan *channel_plan, int max_chunks, bf_batch **batch);
 int bf_batch_destroy(bf_batch *batch);
 
-/* Fixed work-item size in 16-sample tiles (rounded to a multiple of 8).
- * 0 restores the default, guided self-scheduling: items cut largest first,
- * each about (remaining work) / (2 x grid) tiles, so the tail of the queue
- * is fine-grained. */
+/* Fixed work-item size in 16-sample tiles (rounded to a multiple of 8;
+ * items taken longest first). 0 restores the default, balanced items: the
+ * chunks' tile ranges laid end to end and cut into one equal share per
+ * workgroup of the persistent grid, a share crossing chunk boundaries being
+ * run as several segments. */
 int bf_batch_set_item_tiles(bf_batch *batch, int item_tiles);
 
 /* Runs all chunks in one persistent launch with a device-side work queue, on
