@@ -179,9 +179,9 @@ int bf_batch_set_item_tiles(bf_batch *batch, int item_tiles);
  * in the fast range); otherwise BF_ERR_UNSUPPORTED. */
 int bf_batch_run(bf_batch *batch, const bf_chunk *chunks, int nchunks, void *stream);
 
-/* Same work, launched as a replay of the batch's captured hipGraph (queue
- * reset + persistent kernel); only the chunk list and work items are copied
- * per call. */
+/* Same work, launched as a replay of the batch's captured hipGraph (the
+ * persistent kernel); per call, one descriptor copy (chunk list, work items,
+ * zeroed queue head) precedes the replay on `stream`. */
 int bf_batch_run_graph(bf_batch *batch, const bf_chunk *chunks, int nchunks, void *stream);
 
 /* One-line description of the batch configuration. */
