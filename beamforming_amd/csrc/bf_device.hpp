@@ -1,5 +1,6 @@
-// bf_device.hpp -- per-lane arithmetic shared by the gfx950 kernels and the
-// host-side emulator of their data path (bf_emu.cpp).
+// bf_device.hpp -- per-lane arithmetic of the gfx950 kernels. Every function
+// also compiles for the host, where tests/device_math_check.cpp checks it
+// against the oracle.
 //
 // Everything here is integer/byte work except the requantisation, which
 // reproduces the reference's float semantics bit for bit (cpu.cxx:37-39).

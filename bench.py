@@ -13,6 +13,9 @@ Prints one JSON line (stdout) with:
   cpu_baseline   the oracle (CPU restatement of the reference cpu.cxx)
                  on a bounded sample of the same workload on this host
   beam_samples_per_s, hbm_gbps, parity of the sampled frequencies
+  power_and_clock  socket power and engine clocks (amd-smi) while the
+                 full-size launch runs back to back, and the int8 peak at
+                 that clock
 
 Other modes:
   --copy-calib BYTES   run only a plain device copy of BYTES bytes (used to
@@ -113,6 +116,72 @@ def timed(stream, steps, warmup, fn, settle_s=0.0):
     ev1.record(stream)
     torch.cuda.synchronize()
     return ev0.elapsed_time(ev1) / steps
+
+
+def _amd_smi(args):
+    import shutil
+    import subprocess
+    exe = shutil.which("amd-smi") or ("/opt/rocm/bin/amd-smi" if os.path.exists("/opt/rocm/bin/amd-smi") else None)
+    if not exe:
+        return None
+    return subprocess.Popen([exe] + args, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
+
+
+def power_sample(fn, seconds):
+    """Runs fn() back to back for at least `seconds` (and until the sample
+    is in) while `amd-smi metric -p -c` reads the socket power and the engine
+    clock of every XCD once, mid-run; `amd-smi static -l` gives the power
+    limit. Returns None when amd-smi is missing or its output has no power
+    reading. Explains the sustained clock the timed launches ran at."""
+    import re
+    import torch
+    lim = _amd_smi(["static", "-l"])
+    limits = {}
+    if lim is not None:
+        try:
+            text, _ = lim.communicate(timeout=30)
+        except Exception:
+            lim.kill()
+            text = ""
+        for line in text.splitlines():
+            m = re.match(r"\s*([A-Z_]*(?:POWER|CAP)[A-Z_]*):\s*([0-9.]+)\s*W\s*$", line)
+            if m:
+                limits[m.group(1).lower()] = float(m.group(2))
+    settle_clock(fn, 1.0)
+    proc = _amd_smi(["metric", "-p", "-c"])
+    if proc is None:
+        return None
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    n = 0
+    while (proc.poll() is None or time.perf_counter() - t0 < seconds) and time.perf_counter() - t0 < 60:
+        for _ in range(8):
+            fn()
+        torch.cuda.synchronize()
+        n += 8
+    ev1.record()
+    torch.cuda.synchronize()
+    try:
+        text, _ = proc.communicate(timeout=30)
+    except Exception:
+        proc.kill()
+        return None
+    power = re.search(r"SOCKET_POWER:\s*([0-9.]+)\s*W", text)
+    clocks = [float(c) for c in re.findall(r"GFX_\d+:\s*\n\s*CLK:\s*([0-9.]+)\s*MHz", text)]
+    if not power:
+        return None
+    return {
+        "socket_power_w": float(power.group(1)),
+        "power_limits_w": limits or None,
+        "gfx_clock_mhz_mean": round(sum(clocks) / len(clocks), 1) if clocks else None,
+        "gfx_clock_mhz_per_xcd": clocks or None,
+        "gfx_clock_mhz_max": 2400.0,
+        "launches": n,
+        "ms_per_launch": round(ev0.elapsed_time(ev1) / n, 5),
+        "note": "one amd-smi sample taken while the full-size launch ran back to back",
+    }
 
 
 def aligned_offsets(sizes, align=256):
@@ -303,6 +372,8 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="autotune the full-size kernel configuration before timing (default: the built-in choice)")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--power-seconds", type=float, default=3.0,
+                    help="seconds of back-to-back full-size launches while amd-smi samples power and clocks (0: skip)")
     ap.add_argument("--copy-calib", type=int, default=0, help="only run a device copy of this many bytes")
     args = ap.parse_args()
 
@@ -356,6 +427,15 @@ def main():
     gbps = nbytes / (ms * 1e-3) / 1e9
     checksum = bf.adler32_device(dJ.data_ptr(), shape.j_bytes, sptr)
     log(f"full: {ms * 1e3:.1f} us/launch, {tops:.1f} TOPS")
+    power = None
+    if args.power_seconds > 0:
+        power = power_sample(lambda: plan.form_beams_device(dJ.data_ptr(), dE.data_ptr(), sptr), args.power_seconds)
+        if power and power.get("gfx_clock_mhz_mean"):
+            # the dense peak scales with the engine clock (5.0 POP/s at 2.4 GHz)
+            peak_at_clock = MI355X_INT8_TOPS * power["gfx_clock_mhz_mean"] / power["gfx_clock_mhz_max"]
+            power["int8_peak_tops_at_sampled_clock"] = round(peak_at_clock, 1)
+            power["frac_of_peak_at_sampled_clock"] = round(ops / (power["ms_per_launch"] * 1e-3) / 1e12 / peak_at_clock, 4)
+        log(f"power: {power}")
 
     batched = serving = None
     if not args.full_only:
@@ -400,6 +480,7 @@ def main():
         "j_adler32": f"0x{checksum:08x}",
         "wall_s_settle_warmup_and_steps": round(wall, 4),
         "clock_settle_s": args.settle,
+        "power_and_clock": power,
         "batched_vs_plain_launch": batched,
         "serving": serving,
         "cpu_baseline": cpu,
