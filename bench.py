@@ -130,23 +130,11 @@ def _amd_smi(args):
 def power_sample(fn, seconds):
     """Runs fn() back to back for at least `seconds` (and until the sample
     is in) while `amd-smi metric -p -c` reads the socket power and the engine
-    clock of every XCD once, mid-run; `amd-smi static -l` gives the power
-    limit. Returns None when amd-smi is missing or its output has no power
-    reading. Explains the sustained clock the timed launches ran at."""
+    clock of every XCD once, mid-run. Returns None when amd-smi is missing or
+    its output has no power reading. Explains the sustained clock the timed
+    launches ran at."""
     import re
     import torch
-    lim = _amd_smi(["static", "-l"])
-    limits = {}
-    if lim is not None:
-        try:
-            text, _ = lim.communicate(timeout=30)
-        except Exception:
-            lim.kill()
-            text = ""
-        for line in text.splitlines():
-            m = re.match(r"\s*([A-Z_]*(?:POWER|CAP)[A-Z_]*):\s*([0-9.]+)\s*W\s*$", line)
-            if m:
-                limits[m.group(1).lower()] = float(m.group(2))
     settle_clock(fn, 1.0)
     proc = _amd_smi(["metric", "-p", "-c"])
     if proc is None:
@@ -174,7 +162,6 @@ def power_sample(fn, seconds):
         return None
     return {
         "socket_power_w": float(power.group(1)),
-        "power_limits_w": limits or None,
         "gfx_clock_mhz_mean": round(sum(clocks) / len(clocks), 1) if clocks else None,
         "gfx_clock_mhz_per_xcd": clocks or None,
         "gfx_clock_mhz_max": 2400.0,
