@@ -35,6 +35,10 @@ struct bf_batch {
   size_t off_chunks = 16, off_work = 0, off_items = 0, desc_bytes = 0;
   hipEvent_t upload_done = nullptr;
   bool upload_pending = false;
+  // The last launch, so that the next descriptor copy (possibly on another
+  // stream) does not overwrite the block while that launch still reads it.
+  hipEvent_t launch_done = nullptr;
+  bool launch_recorded = false;
   hipStream_t capture_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -180,6 +184,7 @@ int upload(bf_batch *b, const bf_chunk *chunks, int n, hipStream_t st) {
   std::copy(work.begin(), work.end(), reinterpret_cast<int *>(b->h_desc + b->off_work));
   reinterpret_cast<int *>(b->h_desc)[0] = nwork;
   reinterpret_cast<unsigned *>(b->h_desc)[1] = 0; // queue head
+  if (b->launch_recorded) BF_HIP(hipStreamWaitEvent(st, b->launch_done, 0));
   BF_HIP(hipMemcpyAsync(b->d_desc, b->h_desc, b->off_items + items.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
                         st));
   BF_HIP(hipEventRecord(b->upload_done, st));
@@ -216,6 +221,12 @@ int enqueue_launch(bf_batch *b, hipStream_t st) {
   void *args[] = {&a};
   BF_HIP(hipLaunchKernel(reinterpret_cast<const void *>(c.batch_kernel), dim3(b->grid), dim3(kThreads), args,
                          c.lds_bytes, st));
+  return BF_OK;
+}
+
+int record_launch(bf_batch *b, hipStream_t st) {
+  BF_HIP(hipEventRecord(b->launch_done, st));
+  b->launch_recorded = true;
   return BF_OK;
 }
 
@@ -296,6 +307,8 @@ int bf_batch_create(const bf_plan *p, int max_chunks, bf_batch **out) {
   b->grid = p->num_cus * per_cu;
   if ((e = hipEventCreateWithFlags(&b->upload_done, hipEventDisableTiming)) != hipSuccess)
     return bail(hip_fail(e, "hipEventCreate"));
+  if ((e = hipEventCreateWithFlags(&b->launch_done, hipEventDisableTiming)) != hipSuccess)
+    return bail(hip_fail(e, "hipEventCreate"));
   if ((e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking)) != hipSuccess)
     return bail(hip_fail(e, "hipStreamCreate"));
   if (int rc = ensure_item_capacity(b, 4 * b->grid)) return bail(rc);
@@ -306,9 +319,11 @@ int bf_batch_create(const bf_plan *p, int max_chunks, bf_batch **out) {
 int bf_batch_destroy(bf_batch *b) {
   if (!b) return BF_OK;
   if (b->upload_pending) (void)hipEventSynchronize(b->upload_done);
+  if (b->launch_recorded) (void)hipEventSynchronize(b->launch_done); // the launch reads the descriptor block
   free_graph(b);
   if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
   if (b->upload_done) (void)hipEventDestroy(b->upload_done);
+  if (b->launch_done) (void)hipEventDestroy(b->launch_done);
   if (b->d_desc) (void)hipFree(b->d_desc);
   if (b->h_desc) (void)hipHostFree(b->h_desc);
   delete b;
@@ -325,7 +340,8 @@ int bf_batch_run(bf_batch *b, const bf_chunk *chunks, int nchunks, void *stream)
   if (!b) return fail(BF_ERR_INVALID_ARGUMENT, "NULL batch");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (int rc = upload(b, chunks, nchunks, st)) return rc;
-  return enqueue_launch(b, st);
+  if (int rc = enqueue_launch(b, st)) return rc;
+  return record_launch(b, st);
 }
 
 int bf_batch_run_graph(bf_batch *b, const bf_chunk *chunks, int nchunks, void *stream) {
@@ -348,7 +364,7 @@ int bf_batch_run_graph(bf_batch *b, const bf_chunk *chunks, int nchunks, void *s
     BF_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   }
   BF_HIP(hipGraphLaunch(b->exec, st));
-  return BF_OK;
+  return record_launch(b, st);
 }
 
 int bf_batch_describe(const bf_batch *b, char *buf, size_t len) {

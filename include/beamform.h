@@ -175,8 +175,10 @@ int bf_batch_set_item_tiles(bf_batch *batch, int item_tiles);
 
 /* Runs all chunks in one persistent launch with a device-side work queue, on
  * `stream` (asynchronous; the chunk list is copied before the call returns).
- * The plan must use the MFMA path (P = 2, D a multiple of 8 up to 1024, gains
- * in the fast range); otherwise BF_ERR_UNSUPPORTED. */
+ * Runs of one batch are ordered even across streams: each run's descriptor
+ * copy waits for the previous run's launch. Use one batch per concurrent
+ * launch. The plan must use the MFMA path (P = 2, D a multiple of 8 up to
+ * 1024, gains in the fast range); otherwise BF_ERR_UNSUPPORTED. */
 int bf_batch_run(bf_batch *batch, const bf_chunk *chunks, int nchunks, void *stream);
 
 /* Same work, launched as a replay of the batch's captured hipGraph (the

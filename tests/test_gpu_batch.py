@@ -116,6 +116,28 @@ def test_batch_graph_replays_match_eager():
                 torch.cuda.synchronize()
 
 
+def test_batch_runs_on_two_streams_back_to_back():
+    # Two runs of one batch on different streams with no synchronisation in
+    # between: the second run's descriptor copy waits for the first launch.
+    C, D, B = 6, 512, 96
+    A, G = channel_table(C, D, B, 13)
+    shape = bf.Shape.make(1, C, D, B)
+    specs1 = [(20000, 1), (333, 4), (7000, 0)]
+    specs2 = [(50, 2), (9000, 5), (1234, 3), (16, 1)]
+    host1, dev1 = make_chunks(specs1, D, B, 400)
+    host2, dev2 = make_chunks(specs2, D, B, 500)
+    s1 = torch.cuda.Stream()
+    s2 = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with bf.Plan(shape, A, G) as plan, bf.Batch(plan, 8) as batch:
+        say("batch on two streams")
+        batch.run(chunk_list(specs1, dev1), s1.cuda_stream)
+        batch.run(chunk_list(specs2, dev2), s2.cuda_stream)
+        torch.cuda.synchronize()
+    check_chunks(specs1, host1, dev1, A, G, D, B)
+    check_chunks(specs2, host2, dev2, A, G, D, B)
+
+
 def test_batch_item_sizes_and_autotune():
     C, D, B = 4, 512, 96
     A, G = channel_table(C, D, B, 11)
