@@ -4,13 +4,17 @@
 // into per-group device ring buffers, [T_chunk][G][D][P] per slot: the layout
 // the batch and serving paths take (bf_chunk with nfreq = G).
 //
-// The corner turn is done by the copy engines: for each group, one strided
-// 2-D copy per packet segment (source pitch F D P, destination pitch G D P,
-// width G D P bytes), from pinned host memory or from device memory, on the
-// ingest's own stream. A chunk is complete when the packets have filled
-// T_chunk samples; bf_ingest_pop hands it out after making the consumer's
-// stream wait for its copies, and bf_ingest_release returns its slot once
-// the consumer's work on it (ordered on a stream) has finished.
+// The corner turn runs on the ingest's own stream. Device-resident packets
+// (16-byte aligned) go through one scatter kernel per packet segment: every
+// thread moves 16-byte pieces from a packet row to the row of its group's
+// slot, reading and writing whole contiguous rows, so a packet costs one
+// launch whatever the number of groups. Host packets (and unaligned device
+// packets) go through the copy engines: for each group, one strided 2-D copy
+// per packet segment (source pitch F D P, destination pitch G D P, width
+// G D P bytes). A chunk is complete when the packets have filled T_chunk
+// samples; bf_ingest_pop hands it out after making the consumer's stream wait
+// for its copies, and bf_ingest_release returns its slot once the consumer's
+// work on it (ordered on a stream) has finished.
 #include "beamform.h"
 #include "bf_plan.hpp"
 
@@ -35,6 +39,30 @@ struct Ready {
   int group = 0, slot = 0;
 };
 
+// Ring slot of each group of one scatter launch (groups grp0 .. grp0 + n - 1).
+constexpr int kScatterGroups = 256;
+struct SlotPick {
+  uint8_t slot[kScatterGroups];
+};
+
+// Copies rows t = 0 .. n - 1 of a packet segment: group g's width bytes at
+// packet + t spitch + (grp0 + g) width go to its slot at dst_off + t width.
+// width and spitch are multiples of 16, the packet is 16-byte aligned.
+// blockIdx.x covers the 16-byte pieces of a row's groups (row_units =
+// ngroups width / 16 < 2^31), blockIdx.y strides over the rows.
+__global__ void ingest_scatter_kernel(const uint8_t *__restrict__ packet, int64_t n, int64_t spitch, int64_t width,
+                                      int grp0, int ngroups, uint8_t *const *__restrict__ slot_table, int ring_slots,
+                                      SlotPick pick, int64_t dst_off) {
+  const uint32_t wu = (uint32_t)(width / 16);
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= (uint32_t)ngroups * wu) return;
+  const uint32_t g = u / wu, k = u - g * wu;
+  const uint8_t *src = packet + (int64_t)(grp0 + (int)g) * width + 16 * (int64_t)k;
+  uint8_t *dst = slot_table[(int64_t)(grp0 + (int)g) * ring_slots + pick.slot[g]] + dst_off + 16 * (int64_t)k;
+  for (int64_t t = blockIdx.y; t < n; t += gridDim.y)
+    *reinterpret_cast<uint4 *>(dst + t * width) = *reinterpret_cast<const uint4 *>(src + t * spitch);
+}
+
 } // namespace
 
 struct bf_ingest {
@@ -44,6 +72,7 @@ struct bf_ingest {
   size_t slot_bytes = 0;   // T_chunk G D P
   hipStream_t stream = nullptr;
   std::vector<std::vector<Slot>> slots; // [group][slot]
+  uint8_t **d_slot_table = nullptr;     // device: slot base addresses, [group][slot]
   std::vector<int> fill_slot;           // slot being filled, per group (-1: none)
   int64_t fill_pos = 0;                 // samples already in the filling slots (same for every group)
   std::deque<Ready> ready;
@@ -82,6 +111,8 @@ int bf_ingest_create(const bf_ingest_config *cfg, bf_ingest **out) {
     return fail(BF_ERR_INVALID_ARGUMENT, "bad ingest config (P must be 2, G must divide F, at least 2 ring slots, "
                                          "chunk_ntimes >= 1)");
   if (cfg->ndishes % 8 != 0) return fail(BF_ERR_UNSUPPORTED, "ndishes must be a multiple of 8 (16-byte rows)");
+  if (cfg->nfrequencies * cfg->ndishes * cfg->npolarizations / 16 >= (1ll << 31))
+    return fail(BF_ERR_UNSUPPORTED, "packet rows of 32 GiB or more");
   bf_ingest *g = new bf_ingest();
   g->cfg = *cfg;
   g->ngroups = (int)(cfg->nfrequencies / cfg->group_channels);
@@ -104,6 +135,16 @@ int bf_ingest_create(const bf_ingest_config *cfg, bf_ingest **out) {
       if ((e = hipEventCreateWithFlags(&s.released, hipEventDisableTiming)) != hipSuccess)
         return bail(hip_fail(e, "hipEventCreate"));
     }
+  {
+    std::vector<uint8_t *> table;
+    for (auto &grp : g->slots)
+      for (Slot &s : grp) table.push_back(s.E);
+    if ((e = hipMalloc(&g->d_slot_table, table.size() * sizeof(uint8_t *))) != hipSuccess)
+      return bail(hip_fail(e, "hipMalloc (slot table)"));
+    if ((e = hipMemcpy(g->d_slot_table, table.data(), table.size() * sizeof(uint8_t *), hipMemcpyHostToDevice)) !=
+        hipSuccess)
+      return bail(hip_fail(e, "hipMemcpy (slot table)"));
+  }
   *out = g;
   return BF_OK;
 }
@@ -118,6 +159,7 @@ int bf_ingest_destroy(bf_ingest *g) {
       if (s.filled) (void)hipEventDestroy(s.filled);
       if (s.released) (void)hipEventDestroy(s.released);
     }
+  if (g->d_slot_table) (void)hipFree(g->d_slot_table);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
   return BF_OK;
@@ -156,6 +198,15 @@ int bf_ingest_push(bf_ingest *g, const uint8_t *packet, int64_t ntimes, int pack
   }
   // Copy: segments between chunk boundaries.
   const hipMemcpyKind kind = packet_on_host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  // device packets: one scatter kernel per segment (widths are multiples of
+  // 16 because D is a multiple of 8)
+  bool scatter = !packet_on_host && ((uintptr_t)packet & 15u) == 0 && c.ring_slots <= 255;
+  if (scatter) { // only memory the kernel can read for sure: device allocations
+    hipPointerAttribute_t attr;
+    scatter = hipPointerGetAttributes(&attr, packet) == hipSuccess &&
+              (attr.type == hipMemoryTypeDevice || attr.isManaged);
+    (void)hipGetLastError(); // a failed query of a host pointer is not an error here
+  }
   const size_t spitch = (size_t)c.nfrequencies * g->row_bytes;
   const size_t width = (size_t)c.group_channels * g->row_bytes;
   std::vector<size_t> next(g->ngroups, 0);
@@ -169,11 +220,25 @@ int bf_ingest_push(bf_ingest *g, const uint8_t *packet, int64_t ntimes, int pack
         g->slots[grp][s].release_pending = false;
       }
     const int64_t n = std::min(ntimes - done, c.chunk_ntimes - g->fill_pos);
-    for (int grp = 0; grp < g->ngroups; ++grp) {
-      Slot &s = g->slots[grp][g->fill_slot[grp]];
-      BF_HIP(hipMemcpy2DAsync(s.E + (size_t)g->fill_pos * width, width,
-                              packet + (size_t)done * spitch + (size_t)grp * width, spitch, width, (size_t)n, kind,
-                              g->stream));
+    if (scatter) {
+      for (int grp0 = 0; grp0 < g->ngroups; grp0 += kScatterGroups) {
+        const int ng = std::min(kScatterGroups, g->ngroups - grp0);
+        SlotPick pick;
+        for (int i = 0; i < ng; ++i) pick.slot[i] = (uint8_t)g->fill_slot[grp0 + i];
+        const int64_t row_units = (int64_t)ng * (int64_t)(width / 16); // < 2^31, checked at creation
+        const dim3 grid((unsigned)((row_units + 255) / 256), (unsigned)std::min<int64_t>(n, 65535));
+        hipLaunchKernelGGL(ingest_scatter_kernel, grid, dim3(256), 0, g->stream, packet + (size_t)done * spitch,
+                           n, (int64_t)spitch, (int64_t)width, grp0, ng, g->d_slot_table, c.ring_slots, pick,
+                           (int64_t)g->fill_pos * (int64_t)width);
+        BF_HIP(hipGetLastError());
+      }
+    } else {
+      for (int grp = 0; grp < g->ngroups; ++grp) {
+        Slot &s = g->slots[grp][g->fill_slot[grp]];
+        BF_HIP(hipMemcpy2DAsync(s.E + (size_t)g->fill_pos * width, width,
+                                packet + (size_t)done * spitch + (size_t)grp * width, spitch, width, (size_t)n, kind,
+                                g->stream));
+      }
     }
     done += n;
     g->fill_pos += n;

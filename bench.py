@@ -16,6 +16,8 @@ Prints one JSON line (stdout) with:
   power_and_clock  socket power and engine clocks (amd-smi) while the
                  full-size launch runs back to back, and the int8 peak at
                  that clock
+  batched_vs_plain_launch, serving, ingest   the batching, serving and E
+                 ingest paths on their own workloads
 
 Other modes:
   --copy-calib BYTES   run only a plain device copy of BYTES bytes (used to
@@ -238,6 +240,79 @@ def run_batched(bf, bo, torch, stream, steps, warmup, seed=7, C=64, D=512, B=96)
     return res
 
 
+def run_ingest(bf, torch, stream, dE, A, G, T, F, D, B, packet=256, TC=8192, groups=4, slots=4):
+    """E ingest at full size. Packets of `packet` time samples of the resident
+    E[t][f][d][p] are cut into per-group chunks (TC samples x `groups`
+    channels) in device ring slots: (a) alone, each slot released once
+    popped; (b) the F / groups chunks of each chunk time beamformed by one
+    batched launch on the same stream before their slots are released, i.e.
+    the whole telescope pipeline from packets to beams; (c) host-fed packets
+    from pinned memory, alone, on the first TC samples."""
+    sptr = stream.cuda_stream
+    ng = F // groups
+    row = F * D * 2
+    table = bf.Plan(bf.Shape.make(1, F, D, B), A, G)
+    batch = bf.Batch(table, ng)
+    Js = [torch.empty(B * groups * 2 * TC, dtype=torch.uint8, device="cuda") for _ in range(ng)]
+
+    def pump(ing, src, on_host, t_total, beams):
+        t, popped, stalls = 0, [], 0
+        while t < t_total:
+            n = min(packet, t_total - t)
+            if not ing.push(src + t * row, n, on_host):
+                stalls += 1
+                if stalls > 2:
+                    raise RuntimeError("ingest: push refused with every slot released")
+                torch.cuda.synchronize()  # the consumer's releases complete
+                continue
+            stalls = 0
+            t += n
+            while True:
+                r = ing.pop(sptr)
+                if r is None:
+                    break
+                popped.append(r)
+                if len(popped) == ng:
+                    if beams:
+                        batch.run([(c.E, J.data_ptr(), c.ntimes, c.channel, c.nfreq)
+                                   for (c, _), J in zip(popped, Js)], sptr)
+                    for _, sid in popped:
+                        ing.release(sid, sptr)
+                    popped = []
+
+    def timed_pump(src, on_host, t_total, beams):
+        with bf.Ingest(F, D, TC, groups, ring_slots=slots) as ing:
+            pump(ing, src, on_host, TC, beams)  # warm-up: one chunk time
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            pump(ing, src, on_host, t_total, beams)
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0
+
+    nt = T // TC * TC
+    e_bytes = nt * row
+    dt_a = timed_pump(dE.data_ptr(), False, nt, False)
+    dt_b = timed_pump(dE.data_ptr(), False, nt, True)
+    host = torch.empty(TC * row, dtype=torch.uint8, pin_memory=True)
+    host.copy_(dE[:TC * row])
+    dt_c = timed_pump(host.data_ptr(), True, TC, False)
+    batch.close()
+    table.close()
+    return {
+        "workload": f"ingest: full-size E (T={nt} F={F} D={D} P=2) in packets of {packet} samples -> chunks of "
+                    f"{TC} samples x {groups} channels, {slots} ring slots per group",
+        "ingest_only_ms": round(dt_a * 1e3, 3),
+        "ingest_only_gbps": round(e_bytes / dt_a / 1e9, 1),
+        "ingest_only_hbm_frac": round(2 * e_bytes / dt_a / 1e9 / MI355X_HBM_GBPS, 4),
+        "with_beams_ms": round(dt_b * 1e3, 3),
+        "with_beams_beam_samples_per_s": B * F * nt * 2 / dt_b,
+        "host_fed_ingest_only_gbps": round(TC * row / dt_c / 1e9, 1),
+        "host_fed_sample": f"{TC} samples ({TC * row / 2**20:.0f} MiB) from pinned host memory",
+        "note": "wall clock from packet push to the last chunk's release (device synchronised on both sides); "
+                "hbm_frac counts the copy's read and write",
+    }
+
+
 def run_serving(bf, bo, torch, stream, seconds=2.0, rate=2000.0, seed=9, C=64, D=512, B=96,
                 budget=8 << 30, saturation=1000):
     """Poisson arrivals (seeded, `rate` chunks/s) of chunks with nfreq in
@@ -424,12 +499,16 @@ def main():
             power["frac_of_peak_at_sampled_clock"] = round(ops / (power["ms_per_launch"] * 1e-3) / 1e12 / peak_at_clock, 4)
         log(f"power: {power}")
 
-    batched = serving = None
+    batched = serving = ingest = None
     if not args.full_only:
         batched = run_batched(bf, bo, torch, stream, max(5, args.steps // 5), 3)
         log(f"batched: {batched}")
         serving = run_serving(bf, bo, torch, stream, seconds=args.serve_seconds)
         log(f"serving: {serving}")
+        if F % 4 == 0 and T >= 8192:
+            ingest = run_ingest(bf, torch, stream, dE, A, G, T, F, D, B)
+            ingest["plain_launch_beam_samples_per_s"] = B * F * T * 2 / (ms * 1e-3)
+            log(f"ingest: {ingest}")
 
     cpu = None
     if not args.no_cpu:
@@ -470,6 +549,7 @@ def main():
         "power_and_clock": power,
         "batched_vs_plain_launch": batched,
         "serving": serving,
+        "ingest": ingest,
         "cpu_baseline": cpu,
     }
     plan.close()

@@ -26,15 +26,21 @@ def device_bytes(ptr, n):
     return host
 
 
-@pytest.mark.parametrize("on_host", [True, False])
-def test_ingest_chunks_are_packet_slices(on_host):
+@pytest.mark.parametrize("mode", ["host", "device", "device_unaligned"])
+def test_ingest_chunks_are_packet_slices(mode):
+    # device packets go through the scatter kernel; host packets and device
+    # packets that are not 16-byte aligned through 2-D copies
+    on_host = mode == "host"
     F, D, G, TC = 8, 64, 4, 100
     total = 3 * TC + 57
     E = bo.fill_uniform(total * F * D * 2, 71).reshape(total, F, D * 2)
     sizes = [1, 17, 100, 64, 99, 3, 53]  # ragged packets crossing chunk boundaries
     sizes.append(total - sum(sizes))
     assert sizes[-1] > 0
-    dE = torch.from_numpy(E.reshape(-1)).cuda()
+    shift = 8 if mode == "device_unaligned" else 0
+    dE_buf = torch.zeros(E.size + 16, dtype=torch.uint8, device="cuda")
+    dE_buf[shift:shift + E.size].copy_(torch.from_numpy(E.reshape(-1)))
+    base = dE_buf.data_ptr() + shift
     got = []
     with bf.Ingest(F, D, TC, G, ring_slots=4) as ing:
         t = 0
@@ -44,7 +50,7 @@ def test_ingest_chunks_are_packet_slices(on_host):
                 assert ing.push(pkt.ctypes.data, n, True)
                 torch.cuda.synchronize()
             else:
-                assert ing.push(dE.data_ptr() + t * F * D * 2, n, False)
+                assert ing.push(base + t * F * D * 2, n, False)
             t += n
             while True:
                 r = ing.pop(stream())
@@ -93,3 +99,31 @@ def test_ingest_backpressure_and_beams():
         Ek = np.ascontiguousarray(E[k * TC:(k + 1) * TC, ch:ch + G]).reshape(-1)
         want = bo.form_beams(Ek, A.reshape(F, -1)[ch:ch + G], Gn.reshape(F, -1)[ch:ch + G], TC, G, D, B)
         assert np.array_equal(J.cpu().numpy(), want), f"chunk {i}"
+
+
+def test_ingest_many_groups():
+    # 300 one-channel groups: the scatter kernel runs in launches of up to 256 groups
+    F, D, G, TC = 300, 8, 1, 5
+    total = 3 * TC + 2
+    E = bo.fill_uniform(total * F * D * 2, 74).reshape(total, F, D * 2)
+    dE = torch.from_numpy(E.reshape(-1)).cuda()
+    got = []
+    with bf.Ingest(F, D, TC, G, ring_slots=4) as ing:
+        t = 0
+        for n in [4, 7, 6]:  # the 7-sample packet fills the open chunk, one more and starts a third
+            assert ing.push(dE.data_ptr() + t * F * D * 2, n, False)
+            t += n
+            while True:
+                r = ing.pop(stream())
+                if r is None:
+                    break
+                c, sid = r
+                torch.cuda.synchronize()
+                got.append((c.channel, c.ntimes, device_bytes(c.E, TC * G * D * 2)))
+                ing.release(sid, stream())
+        torch.cuda.synchronize()
+    assert len(got) == (total // TC) * F
+    for i, (ch, nt, data) in enumerate(got):
+        k = i // F
+        assert nt == TC and ch == i % F
+        assert np.array_equal(data, E[k * TC:(k + 1) * TC, ch].reshape(-1)), f"chunk {i}"
