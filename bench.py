@@ -12,6 +12,9 @@ Prints one JSON line (stdout) with:
   roofline       dominant kernel vs the MI355X dense int8 peak
   cpu_baseline   the oracle (CPU restatement of the reference cpu.cxx)
                  on a bounded sample of the same workload on this host
+                 (kind "port"), and beside it, under reference_cpu_cxx,
+                 the reference's own cpu.cxx (oracle/_ref, when built) at its
+                 compile-time shape on this host (kind "reference")
   beam_samples_per_s, hbm_gbps, parity of the sampled frequencies
   power_and_clock  socket power and engine clocks (amd-smi) while the
                  full-size launch runs back to back, and the int8 peak at
@@ -82,6 +85,40 @@ def cpu_baseline(E_host, A, G, T, F, D, B, f_sample, J_gpu):
         "seconds": dt,
         "beam_samples_per_s": B * f_sample * T * 2 / dt,
         "parity_vs_gpu": parity,
+    }
+
+
+def cpu_reference(timeout_s=180):
+    """Times the reference's own cpu.cxx form_beams (compiled from its sources
+    into oracle/_ref/ref_harness by oracle/ref.mk, at the reference's shipped
+    compile-time shape) on this host, if that binary was built. Returns None
+    when it is absent or does not run here."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle", "_ref", "ref_harness")
+    if not os.path.exists(exe):
+        return None
+    try:
+        out = subprocess.run([exe, "time"], capture_output=True, text=True, timeout=timeout_s)
+    except (OSError, subprocess.TimeoutExpired):
+        return None
+    lines = [ln for ln in out.stdout.splitlines() if ln.startswith("{")]
+    if out.returncode != 0 or not lines:
+        return None
+    r = json.loads(lines[-1])
+    s = r["shape"]
+    T, F, D, B, P = s["ntimes"], s["nfrequencies"], s["ndishes"], s["nbeams"], s["npolarizations"]
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    return {
+        "value": 8 * T * F * B * D * P / r["seconds"] / 1e12,
+        "unit": "TOPS (int8, algorithmic)",
+        "cores": min(threads, B),
+        "kind": "reference",
+        "sample": f"reference cpu.cxx form_beams (compiled from its sources, its Makefile flags) on setup() inputs "
+                  f"at its compile-time shape T={T} F={F} D={D} P={P} B={B}; {r['seconds']:.2f} s wall around "
+                  f"form_beams only; OpenMP over beams",
+        "seconds": r["seconds"],
+        "beam_samples_per_s": B * F * T * P / r["seconds"],
+        "adler32": f"0x{r['adler32']:08x}",
     }
 
 
@@ -515,6 +552,8 @@ def main():
         log("cpu baseline ...")
         cpu = cpu_baseline(dE.cpu().numpy(), A, G, T, F, D, B, min(args.cpu_frequencies, F), dJ.cpu().numpy())
         log(f"cpu: {cpu['value']:.4f} TOPS in {cpu['seconds']:.2f} s, parity={cpu['parity_vs_gpu']}")
+        cpu["reference_cpu_cxx"] = cpu_reference()
+        log(f"cpu reference: {cpu['reference_cpu_cxx']}")
 
     traffic = load_traffic(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                                         "form_beams_full_traffic.json"))

@@ -14,6 +14,7 @@
 //   ref_harness adler32 <seed> <len>
 //   ref_harness pattern [nsamples]
 //   ref_harness seeded <seed> <nsamples>
+//   ref_harness time      (wall time of one form_beams call on setup() inputs)
 // Each prints one JSON object on stdout.
 #include "adler32.h"
 #include "arraysizes.hxx"
@@ -21,6 +22,7 @@
 
 #include "bf_oracle.h"
 
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -108,6 +110,20 @@ int main(int argc, char **argv) {
     std::printf(", \"mode\": \"%s\", \"seed\": %llu, \"adler32\": %u, ", mode, (unsigned long long)seed, (unsigned)ck);
     print_samples(J, nsamples);
     std::printf("}\n");
+    return 0;
+  }
+  if (!std::strcmp(mode, "time")) {
+    // the reference's timing flow: wall clock around form_beams only (cpu.cxx:58-61)
+    std::vector<ucomplex4> E, A, J;
+    std::vector<float> G;
+    setup(E, A, G, J);
+    const auto t0 = std::chrono::steady_clock::now();
+    form_beams(J.data(), E.data(), A.data(), G.data());
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const uint32_t ck = adler32(reinterpret_cast<const unsigned char *>(J.data()), J.size());
+    std::printf("{");
+    print_shape();
+    std::printf(", \"mode\": \"time\", \"seconds\": %.6f, \"adler32\": %u}\n", dt, (unsigned)ck);
     return 0;
   }
   std::fprintf(stderr, "unknown mode %s\n", mode);
