@@ -90,8 +90,6 @@ int ensure_item_capacity(bf_batch *b, int n) {
   return BF_OK;
 }
 
-int round_up_waves(int64_t t) { return (int)((t + kWaves - 1) / kWaves * kWaves); }
-
 // Work items with a fixed size of `it` tiles, in chunk order.
 void fixed_items(const bf_chunk *chunks, int n, int nbg, int it, std::vector<BatchItem> &items) {
   for (int i = 0; i < n; ++i) {
@@ -104,16 +102,24 @@ void fixed_items(const bf_chunk *chunks, int n, int nbg, int it, std::vector<Bat
   }
 }
 
-// Balanced work items: the (chunk, frequency, beam group) tile ranges are
-// laid end to end and cut into one share of about total / grid tiles (a
-// multiple of the 8 waves) per workgroup; a share that crosses a chunk
-// boundary becomes several segments of one item. Every workgroup then takes
-// one item from the queue, the E pipeline restarts only at segment starts,
-// and the weights are restaged only where a share crosses into another
-// channel.
-void balanced_items(const bf_chunk *chunks, int n, int nbg, int grid, int64_t total, std::vector<BatchItem> &segs,
-                    std::vector<int> &work) {
-  const int64_t share = std::max<int64_t>(kWaves, round_up_waves((total + grid - 1) / grid));
+// Balanced work items. A workgroup runs a segment of m tiles in
+// ceil(m / 8) rounds (one tile per wave per round), so the cost of a
+// (chunk, frequency, beam group) piece is counted in rounds, its last one
+// possibly partial. The pieces are laid end to end and cut into one share of
+// ceil(total rounds / grid) rounds per workgroup; a cut inside a piece falls
+// on a round boundary, and a share that crosses a piece boundary becomes
+// several segments of one item. Every item then costs at most the share in
+// rounds, the E pipeline restarts only at segment starts, and the weights
+// are restaged only where a share crosses into another channel. Returns the
+// share in rounds.
+int64_t balanced_items(const bf_chunk *chunks, int n, int nbg, int grid, std::vector<BatchItem> &segs,
+                       std::vector<int> &work) {
+  int64_t total_rounds = 0;
+  for (int i = 0; i < n; ++i) {
+    const int64_t nt = (chunks[i].ntimes + kTileT - 1) / kTileT;
+    total_rounds += (int64_t)chunk_nfreq(chunks[i]) * nbg * ((nt + kWaves - 1) / kWaves);
+  }
+  const int64_t share = std::max<int64_t>(1, (total_rounds + grid - 1) / grid);
   int64_t fill = 0;
   work.push_back(0);
   for (int i = 0; i < n; ++i) {
@@ -122,10 +128,11 @@ void balanced_items(const bf_chunk *chunks, int n, int nbg, int grid, int64_t to
     for (int f = 0; f < nf; ++f)
       for (int bg = 0; bg < nbg; ++bg)
         for (int64_t t = 0; t < nt;) {
-          const int64_t take = std::min(nt - t, share - fill);
+          const int64_t rounds = std::min((nt - t + kWaves - 1) / kWaves, share - fill);
+          const int64_t take = std::min(nt - t, rounds * kWaves);
           segs.push_back(BatchItem{i, (uint16_t)f, (uint16_t)bg, (int32_t)t, (int32_t)(t + take)});
           t += take;
-          fill += take;
+          fill += rounds;
           if (fill == share) {
             work.push_back((int)segs.size());
             fill = 0;
@@ -133,6 +140,7 @@ void balanced_items(const bf_chunk *chunks, int n, int nbg, int grid, int64_t to
         }
   }
   if (fill > 0) work.push_back((int)segs.size());
+  return share;
 }
 
 // Validates the chunk list, builds the work items (longest first) into the
@@ -142,7 +150,6 @@ int upload(bf_batch *b, const bf_chunk *chunks, int n, hipStream_t st) {
   if (n < 0 || n > b->max_chunks) return fail(BF_ERR_INVALID_ARGUMENT, "nchunks outside [0, max_chunks]");
   if (n > 0 && !chunks) return fail(BF_ERR_INVALID_ARGUMENT, "chunks is NULL");
   const MfmaConfig &c = p->cfg;
-  int64_t total_tiles = 0;
   for (int i = 0; i < n; ++i) {
     const bf_chunk &k = chunks[i];
     const int64_t nf = chunk_nfreq(k);
@@ -152,7 +159,6 @@ int upload(bf_batch *b, const bf_chunk *chunks, int n, hipStream_t st) {
                                                " (NULL buffer, ntimes < 1, or channels outside the table)");
     if (((uintptr_t)k.E & 15u) || ((uintptr_t)k.J & 7u))
       return fail(BF_ERR_INVALID_ARGUMENT, "chunk " + std::to_string(i) + ": E must be 16-byte and J 8-byte aligned");
-    total_tiles += (k.ntimes + kTileT - 1) / kTileT * c.nbg * nf;
   }
   for (int i = 0; i < n; ++i)
     if ((chunks[i].ntimes + kTileT - 1) / kTileT > (1ll << 30)) return fail(BF_ERR_UNSUPPORTED, "chunk too long");
@@ -168,8 +174,7 @@ int upload(bf_batch *b, const bf_chunk *chunks, int n, hipStream_t st) {
     for (int i = 0; i <= (int)items.size(); ++i) work.push_back(i);
     it = b->item_tiles;
   } else {
-    balanced_items(chunks, n, c.nbg, b->grid, total_tiles, items, work);
-    it = (int)std::max<int64_t>(kWaves, round_up_waves((total_tiles + b->grid - 1) / b->grid));
+    it = (int)(balanced_items(chunks, n, c.nbg, b->grid, items, work) * kWaves);
   }
   const int nwork = (int)work.size() - 1;
   if (int rc = ensure_item_capacity(b, std::max<int>(1, (int)items.size()))) return rc;

@@ -169,8 +169,9 @@ int bf_batch_destroy(bf_batch *batch);
 /* Fixed work-item size in 16-sample tiles (rounded to a multiple of 8;
  * items taken longest first). 0 restores the default, balanced items: the
  * chunks' tile ranges laid end to end and cut into one equal share per
- * workgroup of the persistent grid, a share crossing chunk boundaries being
- * run as several segments. */
+ * workgroup of the persistent grid, counted in rounds of 8 tiles (a chunk's
+ * last round may be partial), a share crossing chunk boundaries being run as
+ * several segments. */
 int bf_batch_set_item_tiles(bf_batch *batch, int item_tiles);
 
 /* Runs all chunks in one persistent launch with a device-side work queue, on
