@@ -166,13 +166,21 @@ def _amd_smi(args):
     return subprocess.Popen([exe] + args, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
 
 
+def parse_amd_smi_metric(text):
+    """(socket power in W, [engine clock in MHz per XCD]) from the text of
+    `amd-smi metric -p -c`; power is None when absent."""
+    import re
+    power = re.search(r"SOCKET_POWER:\s*([0-9.]+)\s*W", text)
+    clocks = [float(c) for c in re.findall(r"GFX_\d+:\s*\n\s*CLK:\s*([0-9.]+)\s*MHz", text)]
+    return (float(power.group(1)) if power else None), clocks
+
+
 def power_sample(fn, seconds):
     """Runs fn() back to back for at least `seconds` (and until the sample
     is in) while `amd-smi metric -p -c` reads the socket power and the engine
     clock of every XCD once, mid-run. Returns None when amd-smi is missing or
     its output has no power reading. Explains the sustained clock the timed
     launches ran at."""
-    import re
     import torch
     settle_clock(fn, 1.0)
     proc = _amd_smi(["metric", "-p", "-c"])
@@ -195,12 +203,11 @@ def power_sample(fn, seconds):
     except Exception:
         proc.kill()
         return None
-    power = re.search(r"SOCKET_POWER:\s*([0-9.]+)\s*W", text)
-    clocks = [float(c) for c in re.findall(r"GFX_\d+:\s*\n\s*CLK:\s*([0-9.]+)\s*MHz", text)]
-    if not power:
+    power, clocks = parse_amd_smi_metric(text)
+    if power is None:
         return None
     return {
-        "socket_power_w": float(power.group(1)),
+        "socket_power_w": power,
         "gfx_clock_mhz_mean": round(sum(clocks) / len(clocks), 1) if clocks else None,
         "gfx_clock_mhz_per_xcd": clocks or None,
         "gfx_clock_mhz_max": 2400.0,
