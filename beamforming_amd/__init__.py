@@ -217,6 +217,7 @@ def _load():
         "bf_server_release": (ctypes.c_int, [vp, ctypes.c_uint64]),
         "bf_server_stats": (ctypes.c_int, [vp, ctypes.POINTER(ServerCounters)]),
         "bf_server_clock": (ctypes.c_double, [vp]),
+        "bf_server_update_weights": (ctypes.c_int, [vp, u8p, fp]),
         "bf_ingest_create": (ctypes.c_int, [ctypes.POINTER(IngestConfig), ctypes.POINTER(vp)]),
         "bf_ingest_destroy": (ctypes.c_int, [vp]),
         "bf_ingest_push": (ctypes.c_int, [vp, u8p, ctypes.c_int64, ctypes.c_int]),
@@ -493,6 +494,13 @@ class Server:
 
     def clock(self):
         return lib.bf_server_clock(self._h)
+
+    def update_weights(self, A, G):
+        """New weights and gains (host arrays of the table's shape) for the
+        requests submitted from now on."""
+        A = np.ascontiguousarray(A, dtype=np.uint8)
+        G = np.ascontiguousarray(G, dtype=np.float32)
+        _check(lib.bf_server_update_weights(self._h, A.ctypes.data, G.ctypes.data))
 
 
 class Ingest:

@@ -14,6 +14,12 @@
 //      them to bf_server_poll / bf_server_wait.
 // A result's J stays in the arena until bf_server_release; memory returned
 // there (or by finished E staging) lets the next waiting requests in.
+//
+// Weights are double buffered: bf_server_update_weights prepares a new weight
+// set (a plan of the same shape and its batches) while requests keep running
+// on the current one; requests carry the set that was current when they were
+// submitted, a launch never mixes sets, and a set is freed once none of its
+// requests is pending (at the next update or at shutdown).
 #include "beamform.h"
 #include "bf_plan.hpp"
 #include "bf_sched.hpp"
@@ -45,15 +51,32 @@ struct Request {
   int status = BF_OK;
   int batch_size = 0;
   bool done = false;
+  uint64_t gen = 0; // weight set
 };
 
 struct Launch {
-  bf_batch *batch = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   bool busy = false;
   std::vector<uint64_t> tickets;
 };
+
+// One set of weights: a plan and one batch per launch slot.
+struct WeightSet {
+  uint64_t gen = 0;
+  bf_plan *plan = nullptr;
+  bool owned = false;           // created by the server (else the caller's plan)
+  std::vector<bf_batch *> batches;
+  uint64_t live = 0;            // requests of this set submitted and not finished
+};
+
+void destroy_set(WeightSet &w) {
+  for (bf_batch *b : w.batches)
+    if (b) bf_batch_destroy(b);
+  w.batches.clear();
+  if (w.owned && w.plan) bf_plan_destroy(w.plan);
+  w.plan = nullptr;
+}
 
 } // namespace
 
@@ -65,6 +88,8 @@ struct bf_server {
   ArenaAllocator alloc;
   AdmissionQueue<uint64_t> *adm = nullptr;
   std::vector<Launch> launches;
+  std::deque<WeightSet> sets; // oldest first; back() is current
+  std::mutex update_mu;       // serialises bf_server_update_weights
   std::chrono::steady_clock::time_point epoch;
 
   std::mutex mu;
@@ -88,15 +113,21 @@ namespace {
 
 // Issues one launch: staging copies of host-fed E, then the batched kernel,
 // then the completion event. Called without the server lock.
-int issue(bf_server *s, Launch &L, const std::vector<bf_chunk> &chunks,
+WeightSet *set_of(bf_server *s, uint64_t gen) {
+  for (WeightSet &w : s->sets)
+    if (w.gen == gen) return &w;
+  return nullptr;
+}
+
+int issue(bf_server *s, Launch &L, bf_batch *batch, const std::vector<bf_chunk> &chunks,
           const std::vector<std::pair<uint8_t *, const Request *>> &copies) {
   for (const auto &c : copies) {
     const hipError_t e =
         hipMemcpyAsync(c.first, c.second->E_src, c.second->e_bytes, hipMemcpyHostToDevice, L.stream);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync (E staging)");
   }
-  const int rc = s->cfg.use_graph ? bf_batch_run_graph(L.batch, chunks.data(), (int)chunks.size(), L.stream)
-                                  : bf_batch_run(L.batch, chunks.data(), (int)chunks.size(), L.stream);
+  const int rc = s->cfg.use_graph ? bf_batch_run_graph(batch, chunks.data(), (int)chunks.size(), L.stream)
+                                  : bf_batch_run(batch, chunks.data(), (int)chunks.size(), L.stream);
   if (rc) return rc;
   const hipError_t e = hipEventRecord(L.done, L.stream);
   if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
@@ -117,6 +148,7 @@ void finish(bf_server *s, Launch &L, int rc, double t) {
       r.has_e_piece = false;
     }
     s->adm->finish_slot();
+    if (WeightSet *w = set_of(s, r.gen)) --w->live;
     s->finished.push_back(id);
     ++s->n_completed;
     if (rc) ++s->n_failed;
@@ -162,7 +194,10 @@ void run_loop(bf_server *s) {
       std::vector<bf_chunk> chunks;
       std::vector<std::pair<uint8_t *, const Request *>> copies;
       const double t = s->now();
-      while (!s->ready.empty() && (int)chunks.size() < s->cfg.max_batch) {
+      // one weight set per launch: the oldest waiting request's
+      const uint64_t gen = s->reqs[s->ready.front()].gen;
+      bf_batch *batch = set_of(s, gen)->batches[slot]; // a set with waiting requests is never freed
+      while (!s->ready.empty() && (int)chunks.size() < s->cfg.max_batch && s->reqs[s->ready.front()].gen == gen) {
         const uint64_t id = s->ready.front();
         s->ready.pop_front();
         Request &r = s->reqs[id];
@@ -181,7 +216,7 @@ void run_loop(bf_server *s) {
       s->batch_requests += L.tickets.size();
       s->max_batch_seen = std::max<uint64_t>(s->max_batch_seen, L.tickets.size());
       lk.unlock();
-      const int rc = issue(s, L, chunks, copies); // Request pointers stay valid: entries are only erased on release
+      const int rc = issue(s, L, batch, chunks, copies); // Request pointers stay valid: entries are only erased on release
       lk.lock();
       if (rc) {
         s->loop_error = g_last_error;
@@ -266,8 +301,12 @@ int bf_server_create(const bf_plan *p, const bf_server_config *cfg, bf_server **
   s->alloc.reset(cfg->hbm_budget, 256);
   s->adm = new AdmissionQueue<uint64_t>(&s->alloc, s->cfg.max_inflight);
   s->launches.resize(s->cfg.max_launches);
+  s->sets.emplace_back();
+  s->sets.back().plan = const_cast<bf_plan *>(p); // not owned: never modified or freed here
   for (Launch &L : s->launches) {
-    if (int rc = bf_batch_create(p, s->cfg.max_batch, &L.batch)) return bail(rc);
+    bf_batch *b = nullptr;
+    if (int rc = bf_batch_create(p, s->cfg.max_batch, &b)) return bail(rc);
+    s->sets.back().batches.push_back(b);
     if ((e = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking)) != hipSuccess)
       return bail(hip_fail(e, "hipStreamCreate"));
     if ((e = hipEventCreateWithFlags(&L.done, hipEventDisableTiming)) != hipSuccess)
@@ -288,9 +327,11 @@ int bf_server_destroy(bf_server *s) {
     s->cv_work.notify_all();
     s->loop.join(); // returns once no launch is in flight
   }
-  for (Launch &L : s->launches) {
+  for (Launch &L : s->launches)
     if (L.stream) (void)hipStreamSynchronize(L.stream);
-    if (L.batch) bf_batch_destroy(L.batch);
+  for (WeightSet &w : s->sets) destroy_set(w);
+  s->sets.clear();
+  for (Launch &L : s->launches) {
     if (L.done) (void)hipEventDestroy(L.done);
     if (L.stream) (void)hipStreamDestroy(L.stream);
   }
@@ -339,10 +380,13 @@ int bf_server_submit_many(bf_server *s, const bf_request *reqs, int n, uint64_t 
                                             " bytes, budget is " + std::to_string(s->alloc.capacity()));
       }
     const double t = s->now();
+    WeightSet &cur = s->sets.back();
     for (int i = 0; i < n; ++i) {
       Request &r = rs[i];
       r.ticket = s->next_ticket++;
       r.t_submit = t;
+      r.gen = cur.gen;
+      ++cur.live;
       s->reqs[r.ticket] = r;
       s->adm->push(r.ticket, needs[i]);
       ++s->n_submitted;
@@ -434,5 +478,46 @@ int bf_server_stats(bf_server *s, bf_server_counters *out) {
 }
 
 double bf_server_clock(const bf_server *s) { return s ? s->now() : 0.0; }
+
+int bf_server_update_weights(bf_server *s, const uint8_t *A, const float *G) {
+  if (!s || !A || !G) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> ulk(s->update_mu);
+  hipError_t e = hipSetDevice(s->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  WeightSet w;
+  w.owned = true;
+  // a plan of the same shape: expands the new weights on the device while the
+  // loop keeps launching with the current set
+  if (int rc = bf_plan_create(&s->plan->shape, A, G, &w.plan)) return rc;
+  for (size_t i = 0; i < s->launches.size(); ++i) {
+    bf_batch *b = nullptr;
+    if (int rc = bf_batch_create(w.plan, s->cfg.max_batch, &b)) {
+      destroy_set(w);
+      return rc;
+    }
+    w.batches.push_back(b);
+  }
+  std::vector<WeightSet> retired;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stopping) {
+      destroy_set(w);
+      return fail(BF_ERR_CLOSED, "server is shutting down");
+    }
+    w.gen = s->sets.back().gen + 1;
+    s->sets.push_back(std::move(w));
+    // older sets without pending requests are no longer referenced
+    for (auto it = s->sets.begin(); it + 1 != s->sets.end();) {
+      if (it->live == 0) {
+        retired.push_back(std::move(*it));
+        it = s->sets.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+  for (WeightSet &r : retired) destroy_set(r);
+  return BF_OK;
+}
 
 } // extern "C"

@@ -293,6 +293,18 @@ int bf_server_stats(bf_server *server, bf_server_counters *counters);
 /* Seconds since the server was created (the clock of bf_result). */
 double bf_server_clock(const bf_server *server);
 
+/* Replaces the channel table's weights and gains (host pointers, A [C][B][D]
+ * and G [C][B] like the plan's) for the requests submitted after this call
+ * returns; requests submitted earlier, launched or not, finish with the
+ * weights they were submitted under. The new set is expanded on the device
+ * while the server keeps launching (double buffering, no pause); a set is
+ * freed once none of its requests is pending (at a later update or at
+ * shutdown). The caller's plan is never modified. Synchronous; the new gains
+ * must keep the MFMA path (else BF_ERR_UNSUPPORTED and nothing changes). The
+ * weights/gain update path of the reference's staged variant (cuda4.cu:
+ * 1468-1494 rebuilds A2 per run) as a serving operation. */
+int bf_server_update_weights(bf_server *server, const uint8_t *A, const float *G);
+
 /* ---------------------------------------------------------------------------
  * E ingest: packets of ntimes time samples for all F channels, in the
  * reference layout E[t][f][d][p] (Elinear, arraysizes.hxx:79-89), are cut

@@ -172,3 +172,46 @@ def test_bad_requests_and_shutdown(table):
     t0 = time.time()
     srv.close()
     assert time.time() - t0 < 10
+
+
+def test_weight_updates_while_serving(table):
+    # Requests carry the weights current when they were submitted: three
+    # weight sets, each update issued right after a burst, with the earlier
+    # bursts still queued or running.
+    A0, G0, plan = table
+    sets = [(A0, G0)]
+    for k in (1, 2):
+        sets.append((bo.fill_uniform(C * B * D, 40 + k), bo.setup_gain(C * B, D) * np.float32(4 + 2 * k)))
+    bursts = [random_reqs(10, 60 + k, tmax=8000) for k in range(3)]
+    torch.cuda.synchronize()
+    with bf.Server(plan, hbm_budget=1 << 30) as srv:
+        for k, reqs in enumerate(bursts):
+            if k > 0:
+                srv.update_weights(*sets[k])
+            for i, q in enumerate(reqs):
+                srv.submit(q.dE.data_ptr(), q.T, q.channel, q.nf, tag=100 * k + i)
+        got = collect(srv, sum(len(r) for r in bursts))
+        srv.update_weights(*sets[0])  # retires the finished sets
+        q0 = bursts[0][0]
+        srv.submit(q0.dE.data_ptr(), q0.T, q0.channel, q0.nf, tag=999)
+        again = collect(srv, 1)
+    for k, reqs in enumerate(bursts):
+        A, G = sets[k]
+        for i, q in enumerate(reqs):
+            want = oracle_request(q.E, A, G, q.T, q.channel, q.nf)
+            assert np.array_equal(got[100 * k + i][1], want), f"set {k} request {i}"
+    A, G = sets[0]
+    assert np.array_equal(again[999][1], oracle_request(q0.E, A, G, q0.T, q0.channel, q0.nf))
+
+
+def test_weight_update_refused_outside_mfma_path(table):
+    A, G, plan = table
+    with bf.Server(plan, hbm_budget=1 << 28) as srv:
+        bad = G.copy()
+        bad[0] = np.float32(np.inf)  # the plan would fall back to the generic kernel
+        with pytest.raises(bf.BeamformError):
+            srv.update_weights(A, bad)
+        q = random_reqs(1, 77)[0]
+        srv.submit(q.dE.data_ptr(), q.T, q.channel, q.nf, tag=1)
+        got = collect(srv, 1)
+    assert np.array_equal(got[1][1], oracle_request(q.E, A, G, q.T, q.channel, q.nf))
