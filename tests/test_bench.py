@@ -42,3 +42,12 @@ def test_amd_smi_metric_parsing():
     assert power == 1401.0
     assert clocks == [1613.0, 1535.0]
     assert bench.parse_amd_smi_metric("no reading") == (None, [])
+
+
+def test_synthetic_inputs_match_the_oracle_generators():
+    # bench.py builds its inputs without the oracle; they are the same bytes
+    from oracle import bf_oracle as bo
+    import numpy as np
+    for n, D in [(32 * 96, 512), (64 * 96, 512), (16 * 512, 512), (7, 3)]:
+        assert np.array_equal(bench.reference_gains(n, D), bo.setup_gain(n, D))
+    assert np.array_equal(bench.phase_ramp_weights(16, 512, 512), bo.phase_ramp(16, 512, 512))
