@@ -295,6 +295,8 @@ class Plan:
     def update_weights(self, A, G):
         A_arr, A_p = _host_ptr(A, np.uint8)
         G_arr, G_p = _host_ptr(G, np.float32)
+        if A_arr.size != self.shape.a_bytes or G_arr.size != self.shape.g_count:
+            raise ValueError("A or G size does not match the shape")
         _check(lib.bf_plan_update_weights(self._h, A_p, G_p))
 
     def form_beams_device(self, J_ptr, E_ptr, stream=0):
@@ -498,9 +500,11 @@ class Server:
     def update_weights(self, A, G):
         """New weights and gains (host arrays of the table's shape) for the
         requests submitted from now on."""
-        A = np.ascontiguousarray(A, dtype=np.uint8)
-        G = np.ascontiguousarray(G, dtype=np.float32)
-        _check(lib.bf_server_update_weights(self._h, A.ctypes.data, G.ctypes.data))
+        A_arr, A_p = _host_ptr(A, np.uint8)
+        G_arr, G_p = _host_ptr(G, np.float32)
+        if A_arr.size != self.plan.shape.a_bytes or G_arr.size != self.plan.shape.g_count:
+            raise ValueError("A or G size does not match the channel table")
+        _check(lib.bf_server_update_weights(self._h, A_p, G_p))
 
 
 class Ingest:
