@@ -173,6 +173,28 @@ def test_ragged_shapes(dims, layout):
     check_full(shape, E, A, G)
 
 
+def random_shapes(n, seed, budget=2e8):
+    """n shapes on the MFMA path (D a multiple of 8 up to 1024), T F B D at
+    most `budget`, with a J layout and a gain scale each."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < n:
+        T, F = int(rng.integers(1, 3000)), int(rng.integers(1, 5))
+        D, B = 8 * int(rng.integers(1, 129)), int(rng.integers(1, 301))
+        if T * F * B * D > budget:
+            continue
+        out.append(((T, F, D, B), int(rng.integers(0, 2)), float(rng.choice([1.0, 8.0, 64.0]))))
+    return out
+
+
+@pytest.mark.parametrize("dims,layout,scale", random_shapes(24, 2026))
+def test_random_shapes(dims, layout, scale):
+    T, F, D, B = dims
+    shape = bf.Shape.make(T, F, D, B, j_layout=layout)
+    E, A, G = seeded(T, F, D, B, T * 131 + D * 7 + B)
+    check_full(shape, E, A, G * np.float32(scale))
+
+
 @pytest.mark.parametrize("dims,P", [((7, 2, 5, 3), 2), ((9, 1, 16, 4), 1), ((5, 2, 12, 6), 3), ((6, 1, 1100, 3), 2)])
 def test_generic_kernel_shapes(dims, P):
     T, F, D, B = dims
