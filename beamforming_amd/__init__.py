@@ -204,6 +204,7 @@ def _load():
         "bf_batch_run_graph": (ctypes.c_int, [vp, ctypes.POINTER(Chunk), ctypes.c_int, vp]),
         "bf_batch_describe": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_size_t]),
         "bf_plan_autotune": (ctypes.c_int, [vp, u8p, u8p, vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]),
+        "bf_plan_configure": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
         "bf_batch_autotune": (ctypes.c_int, [vp, ctypes.POINTER(Chunk), ctypes.c_int, vp, ctypes.c_int,
                                              ctypes.c_char_p, ctypes.c_size_t]),
         "bf_server_create": (ctypes.c_int, [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
@@ -320,6 +321,10 @@ class Plan:
         _check(lib.bf_plan_autotune(self._h, ctypes.c_void_p(J_ptr), ctypes.c_void_p(E_ptr), ctypes.c_void_p(stream),
                                     iters, buf, 4096))
         return buf.value.decode()
+
+    def configure(self, beam_tiles=0, stage_j=True, wg_scale=1.0):
+        """Sets the kernel configuration explicitly (see bf_plan_configure)."""
+        _check(lib.bf_plan_configure(self._h, int(beam_tiles), int(bool(stage_j)), float(wg_scale)))
 
     def form_beams_host(self, E):
         """Synchronous: host E (numpy uint8) -> host J (numpy uint8)."""

@@ -134,11 +134,19 @@ bool choose_tiling(int64_t ndishes, int64_t nbeams, int forced_nt, MfmaConfig *c
   cfg->nt = best_nt;
   cfg->nbg = best_nbg;
   cfg->ntp = best_nt * best_nbg;
-  cfg->lds_bytes = mfma_lds_bytes(best_nt, ks);
+  choose_lds_layout(0, true, cfg);
   cfg->dish_tail = (int64_t)ks * kDishesPerStep != ndishes;
   cfg->kernel = lookup_mfma(best_nt, ks, cfg->dish_tail);
   cfg->batch_kernel = lookup_batch(best_nt, ks, cfg->dish_tail);
   return cfg->kernel != nullptr && cfg->batch_kernel != nullptr;
+}
+
+// LDS layout: the J staging area is used with the [b][f][p][t] layout when it
+// fits beside the weights (and when wanted); without it the workgroup needs
+// less LDS, which can let more workgroups share a CU.
+void choose_lds_layout(int j_layout, bool want_staging, MfmaConfig *cfg) {
+  cfg->stage_j = want_staging && j_layout == BF_J_BFPT && mfma_stages_j(cfg->nt, cfg->ks);
+  cfg->lds_bytes = mfma_lds_unstaged(cfg->nt, cfg->ks) + (cfg->stage_j ? mfma_lds_staging(cfg->nt) : 0);
 }
 
 // Grid for T time samples and F frequencies: about target_wgs workgroups, each
@@ -177,8 +185,10 @@ bool choose_mfma_config(const bf_shape &s, int num_cus, MfmaConfig *cfg, const T
   TuneChoice tc;
   const bool tuned = force ? (tc = *force, true) : recall_tuning(s, &tc);
   if (!choose_tiling(s.ndishes, s.nbeams, tuned ? tc.nt : 0, cfg)) return false;
+  choose_lds_layout(s.j_layout, tuned ? tc.stage_j : true, cfg);
+  // the instantiation's largest layout, so plans of either layout can share it
   if (hipFuncSetAttribute(reinterpret_cast<const void *>(cfg->kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)cfg->lds_bytes) != hipSuccess)
+                          (int)mfma_lds_bytes(cfg->nt, cfg->ks)) != hipSuccess)
     return false;
   const double scale = tuned ? tc.wg_scale : 1.0;
   const int64_t target = (int64_t)std::llround(num_cus * wgs_per_cu(*cfg) * scale);
@@ -314,6 +324,7 @@ int launch_mfma(const bf_plan *p, const MfmaConfig &c, uint8_t *J, const uint8_t
   a.per_xcd = c.per_xcd;
   a.jlayout = s.j_layout;
   a.wrow0 = wrow0;
+  a.stage_j = c.stage_j ? 1 : 0;
   void *args[] = {&a};
   BF_HIP(hipLaunchKernel(reinterpret_cast<const void *>(c.kernel), dim3(c.grid), dim3(kThreads), args, c.lds_bytes, st));
   return BF_OK;
@@ -403,9 +414,10 @@ int bf_plan_describe(const bf_plan *p, char *buf, size_t len) {
   if (p->use_mfma) {
     const MfmaConfig &c = p->cfg;
     std::snprintf(buf, len,
-                  "mfma_i32_32x32x32_i8 nt=%d ks=%d nbg=%d tile_t=%d waves=%d tiles_per_chunk=%d nchunks=%d grid=%d "
-                  "lds=%zu",
-                  c.nt, c.ks, c.nbg, kTileT, kWaves, c.tiles_per_chunk, c.nchunks, c.grid, c.lds_bytes);
+                  "mfma_i32_32x32x32_i8 nt=%d ks=%d nbg=%d stage_j=%d tile_t=%d waves=%d tiles_per_chunk=%d nchunks=%d "
+                  "grid=%d lds=%zu",
+                  c.nt, c.ks, c.nbg, c.stage_j ? 1 : 0, kTileT, kWaves, c.tiles_per_chunk, c.nchunks, c.grid,
+                  c.lds_bytes);
   } else {
     std::snprintf(buf, len, "generic (%s)", p->fast_gains ? "shape outside MFMA range" : "gain outside fast range");
   }

@@ -215,6 +215,7 @@ BatchParams batch_params(const bf_batch *b) {
   a.D = (int32_t)p->shape.ndishes;
   a.B = (int32_t)p->shape.nbeams;
   a.jlayout = p->shape.j_layout;
+  a.stage_j = p->cfg.stage_j ? 1 : 0;
   return a;
 }
 
@@ -301,8 +302,10 @@ int bf_batch_create(const bf_plan *p, int max_chunks, bf_batch **out) {
   };
   const MfmaConfig &c = p->cfg;
   hipError_t e;
+  // the instantiation's largest layout, so plans of either layout can share it
   if ((e = hipFuncSetAttribute(reinterpret_cast<const void *>(c.batch_kernel),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds_bytes)) != hipSuccess)
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)mfma_lds_bytes(c.nt, c.ks))) !=
+      hipSuccess)
     return bail(hip_fail(e, "hipFuncSetAttribute(batch)"));
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(c.batch_kernel), kThreads,
@@ -376,10 +379,11 @@ int bf_batch_describe(const bf_batch *b, char *buf, size_t len) {
   if (!b || !buf || len == 0) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
   const MfmaConfig &c = b->plan->cfg;
   std::snprintf(buf, len,
-                "persistent mfma_i32_32x32x32_i8 batch: nt=%d ks=%d nbg=%d grid=%d lds=%zu item_tiles=%d%s items=%d "
+                "persistent mfma_i32_32x32x32_i8 batch: nt=%d ks=%d nbg=%d stage_j=%d grid=%d lds=%zu item_tiles=%d%s "
+                "items=%d "
                 "segments=%d "
                 "graph=%s",
-                c.nt, c.ks, c.nbg, b->grid, c.lds_bytes, b->last_item_tiles, b->item_tiles ? "" : " (balanced)",
+                c.nt, c.ks, c.nbg, c.stage_j ? 1 : 0, b->grid, c.lds_bytes, b->last_item_tiles, b->item_tiles ? "" : " (balanced)",
                 b->last_nitems, b->last_segments, b->exec ? "captured" : "none");
   return BF_OK;
 }
@@ -403,24 +407,29 @@ int bf_plan_autotune(bf_plan *p, uint8_t *J, const uint8_t *E, void *stream, int
     MfmaConfig t;
     if (!choose_tiling(s.ndishes, s.nbeams, nt, &t)) continue;
     if (3 * t.ntp > 4 * min_ntp) continue; // pads a third more beam tiles than the best tiling
-    for (double scale : {0.5, 1.0, 2.0}) {
-      const TuneChoice choice{nt, scale};
-      MfmaConfig c;
-      if (!choose_mfma_config(s, p->num_cus, &c, &choice)) continue;
-      p->cfg = c;
-      if (int rc = alloc_weight_tables(p)) return rc;
-      if (int rc = prepare_weights(p, st)) return rc;
-      float ms = 0;
-      if (int rc = time_launches(st, iters, &ms, [&] { return launch_form_beams(p, J, E, st); })) return rc;
-      char line[160];
-      std::snprintf(line, sizeof line, "nt=%d nbg=%d scale=%.1f grid=%d: %.1f us\n", c.nt, c.nbg, scale, c.grid,
-                    ms * 1e3f);
-      rep += line;
-      if (ms < best_ms) {
-        best_ms = ms;
-        best = choice;
+    // LDS layout: with the J staging area where it fits, and without it
+    MfmaConfig probe = t;
+    choose_lds_layout(s.j_layout, true, &probe);
+    const int layouts = probe.stage_j ? 2 : 1;
+    for (int layout = 0; layout < layouts; ++layout)
+      for (double scale : {0.5, 1.0, 2.0}) {
+        const TuneChoice choice{nt, scale, layout == 0};
+        MfmaConfig c;
+        if (!choose_mfma_config(s, p->num_cus, &c, &choice)) continue;
+        p->cfg = c;
+        if (int rc = alloc_weight_tables(p)) return rc;
+        if (int rc = prepare_weights(p, st)) return rc;
+        float ms = 0;
+        if (int rc = time_launches(st, iters, &ms, [&] { return launch_form_beams(p, J, E, st); })) return rc;
+        char line[160];
+        std::snprintf(line, sizeof line, "nt=%d nbg=%d stage_j=%d scale=%.1f grid=%d lds=%zu: %.1f us\n", c.nt, c.nbg,
+                      c.stage_j ? 1 : 0, scale, c.grid, c.lds_bytes, ms * 1e3f);
+        rep += line;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = choice;
+        }
       }
-    }
   }
   MfmaConfig c;
   if (!choose_mfma_config(s, p->num_cus, &c, &best)) return fail(BF_ERR_UNSUPPORTED, "no candidate configuration");
@@ -430,10 +439,26 @@ int bf_plan_autotune(bf_plan *p, uint8_t *J, const uint8_t *E, void *stream, int
   if (int rc = prepare_weights(p, st)) return rc;
   remember_tuning(s, best);
   char line[160];
-  std::snprintf(line, sizeof line, "chosen: nt=%d scale=%.1f (%.1f us)\n", best.nt, best.wg_scale, best_ms * 1e3f);
+  std::snprintf(line, sizeof line, "chosen: nt=%d stage_j=%d scale=%.1f (%.1f us)\n", best.nt, c.stage_j ? 1 : 0,
+                best.wg_scale, best_ms * 1e3f);
   rep += line;
   if (report && report_len) std::snprintf(report, report_len, "%s", rep.c_str());
   return BF_OK;
+}
+
+int bf_plan_configure(bf_plan *p, int beam_tiles, int stage_j, double wg_scale) {
+  if (!p) return fail(BF_ERR_INVALID_ARGUMENT, "NULL plan");
+  if (!p->use_mfma) return fail(BF_ERR_UNSUPPORTED, "the plan does not use the MFMA kernel");
+  if (beam_tiles < 0 || beam_tiles > 4 || (stage_j != 0 && stage_j != 1) || !(wg_scale > 0.0) || wg_scale > 64.0)
+    return fail(BF_ERR_INVALID_ARGUMENT, "beam_tiles in 0..4, stage_j 0 or 1, wg_scale in (0, 64]");
+  const TuneChoice choice{beam_tiles, wg_scale, stage_j != 0};
+  MfmaConfig c;
+  if (!choose_mfma_config(p->shape, p->num_cus, &c, &choice))
+    return fail(BF_ERR_UNSUPPORTED, "no MFMA configuration with these beam tiles for the shape");
+  p->cfg = c;
+  p->wgs_per_cu = wgs_per_cu(c);
+  if (int rc = alloc_weight_tables(p)) return rc;
+  return prepare_weights(p, nullptr);
 }
 
 int bf_batch_autotune(bf_batch *b, const bf_chunk *chunks, int nchunks, void *stream, int iters, char *report,

@@ -182,6 +182,7 @@ struct MfmaParams {
   int32_t per_xcd;   // ceil(nwork / 8)
   int32_t jlayout;   // 0 = [b][f][p][t], 1 = [b][f][t][p]
   int32_t wrow0;     // weight-table row of frequency 0 (a channel, for one-channel calls)
+  int32_t stage_j;   // 1: J rows go through the LDS staging area (allocated in the launch's LDS)
 };
 
 // Work index of a workgroup. Workgroups are dispatched round-robin over the

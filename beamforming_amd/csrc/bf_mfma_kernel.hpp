@@ -87,8 +87,9 @@ __device__ __forceinline__ void store_row8(uint8_t *dst, uint32_t w0, uint32_t w
 
 // LDS of a workgroup: [KS][NT][64] weight operands, the beam-column nibble
 // offsets and gains of the beam group, one 1 KiB E transpose buffer per wave,
-// then, when it fits in the 160 KiB of a CU, the J staging area: 2 rounds x
-// (NT x 32 beams x 2 polarizations) rows x 128 bytes.
+// then, when the launch stages J (the [b][f][p][t] layout, and it fits in the
+// 160 KiB of a CU), the J staging area: 2 rounds x (NT x 32 beams x 2
+// polarizations) rows x 128 bytes.
 constexpr size_t kLdsPerCu = 160 * 1024;
 BF_HD constexpr size_t mfma_lds_unstaged(int nt, int ks) {
   return (size_t)ks * nt * 64 * sizeof(v4i) + (size_t)nt * 32 * (sizeof(int2) + sizeof(float)) +
@@ -409,7 +410,7 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
   job.B = a.B;
   job.bg = wi.bg;
   job.jlayout = a.jlayout;
-  v4i *Sl = mfma_stages_j(NT, KS) && a.jlayout == 0 ? Xl + kWaves * 64 : nullptr;
+  v4i *Sl = a.stage_j ? Xl + kWaves * 64 : nullptr;
   run_tiles<NT, KS, DT>(job, tile_begin, tile_end, Wl, Cl, Gl, Xl + wave * 64, Sl);
 }
 
@@ -443,6 +444,7 @@ struct BatchParams {
   const int *corr;    // [C][BP][2]
   const float *G;     // [C][BP]
   int32_t NTP, D, B, jlayout;
+  int32_t stage_j; // 1: J rows go through the LDS staging area (allocated in the launch's LDS)
 };
 
 template <int NT, int KS, bool DT>
@@ -453,7 +455,7 @@ __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a
   v4i *Xl = reinterpret_cast<v4i *>(Gl + NT * 32);
   __shared__ int s_item;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  v4i *Sl = mfma_stages_j(NT, KS) && a.jlayout == 0 ? Xl + kWaves * 64 : nullptr;
+  v4i *Sl = a.stage_j ? Xl + kWaves * 64 : nullptr;
   int loaded_channel = -1, loaded_bg = -1;
   const int nitems = *a.nitems;
   for (;;) {

@@ -20,6 +20,7 @@ struct MfmaConfig {
   int nwork = 0;           // workgroups with work
   int per_xcd = 0;         // work items per XCD
   int grid = 0;            // launched workgroups (8 * per_xcd)
+  bool stage_j = false;    // J rows staged in LDS (whole-line stores); else direct 16-byte stores
   size_t lds_bytes = 0;    // dynamic LDS per workgroup
   MfmaKernel kernel = nullptr;
   BatchKernel batch_kernel = nullptr;
@@ -30,6 +31,7 @@ struct MfmaConfig {
 struct TuneChoice {
   int nt = 0;
   double wg_scale = 1.0;
+  bool stage_j = true; // stage J rows in LDS when the layout and the LDS allow it
 };
 
 } // namespace bf
@@ -56,6 +58,7 @@ int hip_fail(hipError_t e, const char *what);
 int check_shape(const bf_shape *s);
 bool choose_mfma_config(const bf_shape &s, int num_cus, MfmaConfig *cfg, const TuneChoice *force);
 bool choose_tiling(int64_t ndishes, int64_t nbeams, int forced_nt, MfmaConfig *cfg);
+void choose_lds_layout(int j_layout, bool want_staging, MfmaConfig *cfg);
 bool choose_grid(int64_t ntimes, int64_t nfrequencies, int64_t target_wgs, MfmaConfig *cfg);
 int wgs_per_cu(const MfmaConfig &cfg);
 void remember_tuning(const bf_shape &s, const TuneChoice &c);

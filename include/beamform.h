@@ -190,11 +190,20 @@ int bf_batch_run_graph(bf_batch *batch, const bf_chunk *chunks, int nchunks, voi
 /* One-line description of the batch configuration. */
 int bf_batch_describe(const bf_batch *batch, char *buf, size_t len);
 
-/* Times candidate kernel configurations (beam tiles per workgroup, grid size)
- * on the caller's device buffers E and J with the plan's shape, keeps the
- * fastest in the plan, and remembers it for later plans of the same shape in
- * this process. `iters` timed launches per candidate (0 = default). A report
- * of the candidates is written to report (may be NULL). Synchronous. */
+/* Times candidate kernel configurations (beam tiles per workgroup, LDS
+ * layout: J rows staged in LDS for whole-line stores or stored directly, and
+ * the number of workgroups, i.e. the time split) on the caller's device
+ * buffers E and J with the plan's shape, keeps the fastest in the plan, and
+ * remembers it for later plans of the same shape in this process. `iters`
+ * timed launches per candidate (0 = default). A report of the candidates is
+ * written to report (may be NULL). Synchronous. */
+/* Sets the kernel configuration explicitly, as the autotuner does after
+ * timing: beam_tiles 32-beam tiles per workgroup (1..4, 0 = automatic),
+ * stage_j 1 to stage J rows in LDS where the layout and the LDS allow it,
+ * wg_scale times the default number of workgroups. Work already enqueued
+ * with the plan must have finished. Synchronous. */
+int bf_plan_configure(bf_plan *plan, int beam_tiles, int stage_j, double wg_scale);
+
 int bf_plan_autotune(bf_plan *plan, uint8_t *J, const uint8_t *E, void *stream, int iters, char *report,
                      size_t report_len);
 

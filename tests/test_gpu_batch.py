@@ -198,6 +198,37 @@ def test_plan_autotune_keeps_results_exact():
     assert np.array_equal(dJ.cpu().numpy(), bo.form_beams(E, A, G, T, F, D, B))
 
 
+@pytest.mark.parametrize("nt,stage", [(3, True), (3, False), (1, True), (1, False), (2, False), (4, False)])
+def test_configured_layouts_are_exact(nt, stage):
+    # every beam tiling and LDS layout the autotuner can pick, plain and batched
+    T, F, D, B = 1000, 3, 512, 96
+    shape = bf.Shape.make(T, F, D, B)
+    E = bo.fill_uniform(T * F * D * 2, 23)
+    A = bo.fill_uniform(F * B * D, 24)
+    G = bo.setup_gain(F * B, D) * np.float32(8)
+    want = bo.form_beams(E, A, G, T, F, D, B)
+    dE = torch.from_numpy(E).cuda()
+    dJ = torch.zeros(shape.j_bytes, dtype=torch.uint8, device="cuda")
+    with bf.Plan(shape, A, G) as plan:
+        plan.configure(nt, stage, 1.0)
+        desc = plan.describe()
+        say(desc)
+        assert f"nt={nt} " in desc and f"stage_j={int(stage and nt < 4)}" in desc
+        plan.form_beams_device(dJ.data_ptr(), dE.data_ptr(), stream())
+        torch.cuda.synchronize()
+    assert np.array_equal(dJ.cpu().numpy(), want)
+    # the same configuration through the batch path (channel table of the F channels)
+    specs = [(T, c) for c in range(F)] + [(77, 1)]
+    host, dev = make_chunks(specs, D, B, 600)
+    with bf.Plan(bf.Shape.make(1, F, D, B), A, G) as table:
+        table.configure(nt, stage, 1.0)
+        with bf.Batch(table, 8) as batch:
+            batch.run(chunk_list(specs, dev), stream())
+            torch.cuda.synchronize()
+            say(batch.describe())
+    check_chunks(specs, host, dev, A, G, D, B)
+
+
 @pytest.mark.parametrize("layout", [bf.BF_J_BFPT, bf.BF_J_BFTP])
 def test_multi_channel_plain_and_batched(layout):
     C, D, B = 20, 512, 96
