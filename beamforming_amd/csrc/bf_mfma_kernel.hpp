@@ -22,11 +22,12 @@
 //    (cpu.cxx:37-39), packs ucomplex4 bytes and swaps halves between lanes
 //    l and l + 32 (v_permlane32_swap), so that each lane holds the 16 bytes
 //    of 16 consecutive times of one (beam, polarization) row;
-//  * the 8 waves advance in rounds of 8 consecutive tiles. With the
-//    [b][f][p][t] layout and room in LDS, each wave writes its rows into a
-//    double-buffered LDS staging area and, after the round's barrier, the
-//    workgroup stores every row's 128 bytes (8 tiles) with whole-line
-//    writes; otherwise each lane stores its 16 bytes directly.
+//  * the 8 waves advance in rounds of 8 consecutive tiles. When the launch
+//    stages J (the [b][f][p][t] layout, room in LDS, and the configuration
+//    asks for it), each wave writes its rows into a double-buffered LDS
+//    staging area and, after the round's barrier, the workgroup stores every
+//    row's 128 bytes (8 tiles) with whole-line writes; otherwise each lane
+//    stores its 16 bytes directly.
 // Rows past T and dishes past D are clamped, always-in-bounds loads: the
 // outputs of rows past T are never stored and the weights of dishes past D
 // are zero. The plan
