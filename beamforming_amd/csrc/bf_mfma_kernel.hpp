@@ -411,7 +411,7 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
   job.B = a.B;
   job.bg = wi.bg;
   job.jlayout = a.jlayout;
-  v4i *Sl = a.stage_j ? Xl + kWaves * 64 : nullptr;
+  v4i *Sl = mfma_stages_j(NT, KS) && a.stage_j ? Xl + kWaves * 64 : nullptr; // compiled out where it cannot fit
   run_tiles<NT, KS, DT>(job, tile_begin, tile_end, Wl, Cl, Gl, Xl + wave * 64, Sl);
 }
 
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a
   v4i *Xl = reinterpret_cast<v4i *>(Gl + NT * 32);
   __shared__ int s_item;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  v4i *Sl = a.stage_j ? Xl + kWaves * 64 : nullptr;
+  v4i *Sl = mfma_stages_j(NT, KS) && a.stage_j ? Xl + kWaves * 64 : nullptr; // compiled out where it cannot fit
   int loaded_channel = -1, loaded_bg = -1;
   const int nitems = *a.nitems;
   for (;;) {
