@@ -326,6 +326,11 @@ class Plan:
         """Sets the kernel configuration explicitly (see bf_plan_configure)."""
         _check(lib.bf_plan_configure(self._h, int(beam_tiles), int(bool(stage_j)), float(wg_scale)))
 
+    def form_beams_host_ptr(self, J_ptr, E_ptr):
+        """Synchronous copy-in / run / copy-out on host buffers given as raw
+        pointers (E: e_bytes, J: j_bytes)."""
+        _check(lib.bf_form_beams_host(self._h, ctypes.c_void_p(J_ptr), ctypes.c_void_p(E_ptr)))
+
     def form_beams_host(self, E):
         """Synchronous: host E (numpy uint8) -> host J (numpy uint8)."""
         E_arr, E_p = _host_ptr(E, np.uint8)

@@ -19,6 +19,8 @@ Prints one JSON line (stdout) with:
   power_and_clock  socket power and engine clocks (amd-smi) while the
                  full-size launch runs back to back, and the int8 peak at
                  that clock
+  host_path      the copy-in / launch / copy-out entry point on pinned host
+                 buffers (PCIe-inclusive, never the value)
   other_configs  the small and wide configurations, timed the same way
   batched_vs_plain_launch, serving, ingest   the batching, serving and E
                  ingest paths on their own workloads
@@ -306,6 +308,26 @@ def run_batched(bf, torch, stream, steps, warmup, seed=7, C=64, D=512, B=96):
     batch.close()
     plan.close()
     return res
+
+
+def run_host_path(bf, torch, plan, dE, shape, reps=3):
+    """bf_form_beams_host (copy E in, launch, copy J out; the flow of the
+    reference's cuda.cu) on pinned host buffers holding the same E: the
+    PCIe-inclusive rate of a caller that hands over host memory."""
+    hE = torch.empty(shape.e_bytes, dtype=torch.uint8, pin_memory=True)
+    hJ = torch.empty(shape.j_bytes, dtype=torch.uint8, pin_memory=True)
+    hE.copy_(dE)
+    plan.form_beams_host_ptr(hJ.data_ptr(), hE.data_ptr())  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.form_beams_host_ptr(hJ.data_ptr(), hE.data_ptr())
+    dt = (time.perf_counter() - t0) / reps
+    return {
+        "ms_per_call": round(dt * 1e3, 3),
+        "pcie_gbps": round((shape.e_bytes + shape.j_bytes) / dt / 1e9, 1),
+        "beam_samples_per_s": shape.j_bytes / dt,
+        "note": "pinned host E and J; includes device allocation, the E copy in and the J copy out",
+    }
 
 
 def run_other_configs(bf, torch, stream, steps, warmup):
@@ -598,8 +620,10 @@ def main():
             power["frac_of_peak_at_sampled_clock"] = round(ops / (power["ms_per_launch"] * 1e-3) / 1e12 / peak_at_clock, 4)
         log(f"power: {power}")
 
-    batched = serving = ingest = others = None
+    batched = serving = ingest = others = host_path = None
     if not args.full_only:
+        host_path = run_host_path(bf, torch, plan, dE, shape)
+        log(f"host path: {host_path}")
         others = run_other_configs(bf, torch, stream, args.steps, args.warmup)
         log(f"other configurations: {others}")
         batched = run_batched(bf, torch, stream, max(5, args.steps // 5), 3)
@@ -650,6 +674,7 @@ def main():
         "wall_s_settle_warmup_and_steps": round(wall, 4),
         "clock_settle_s": args.settle,
         "power_and_clock": power,
+        "host_path": host_path,
         "other_configs": others,
         "batched_vs_plain_launch": batched,
         "serving": serving,
