@@ -215,3 +215,33 @@ def test_weight_update_refused_outside_mfma_path(table):
         srv.submit(q.dE.data_ptr(), q.T, q.channel, q.nf, tag=1)
         got = collect(srv, 1)
     assert np.array_equal(got[1][1], oracle_request(q.E, A, G, q.T, q.channel, q.nf))
+
+
+def test_submitters_on_several_threads(table):
+    # four host threads submit at once (the library takes its lock; ctypes
+    # releases the GIL during each call) while the main thread collects
+    import threading
+    A, G, plan = table
+    groups = [random_reqs(8, 90 + k, tmax=3000) for k in range(4)]
+    torch.cuda.synchronize()
+    errors = []
+    with bf.Server(plan, hbm_budget=1 << 29, max_inflight=16) as srv:
+        def submit_all(k):
+            try:
+                for i, q in enumerate(groups[k]):
+                    srv.submit(q.dE.data_ptr(), q.T, q.channel, q.nf, tag=100 * k + i)
+            except Exception as exc:  # reported below
+                errors.append(exc)
+        threads = [threading.Thread(target=submit_all, args=(k,)) for k in range(4)]
+        for t in threads:
+            t.start()
+        got = collect(srv, 32)
+        for t in threads:
+            t.join()
+        st = srv.stats()
+    assert not errors, errors
+    assert st["submitted"] == 32 and st["completed"] == 32 and st["bytes_in_use"] == 0
+    for k, reqs in enumerate(groups):
+        for i, q in enumerate(reqs):
+            want = oracle_request(q.E, A, G, q.T, q.channel, q.nf)
+            assert np.array_equal(got[100 * k + i][1], want), f"thread {k} request {i}"
