@@ -48,6 +48,10 @@ int check_shape(const bf_shape *s) {
   // int32 accumulation is exact for |raw| <= 128 D < 2^31 and the int -> float
   // conversion of raw is exact below 2^24 (as in the reference).
   if (s->ndishes > 65536) return fail(BF_ERR_UNSUPPORTED, "ndishes > 65536");
+  // byte counts of E and J (and time tile indices) stay far inside int64
+  const double nE = (double)s->ntimes * (double)s->nfrequencies * (double)s->ndishes * (double)s->npolarizations;
+  const double nJ = (double)s->nbeams * (double)s->nfrequencies * (double)s->npolarizations * (double)s->ntimes;
+  if (s->ntimes > kMaxTimes || nE > 0x1p62 || nJ > 0x1p62) return fail(BF_ERR_UNSUPPORTED, "E or J too large");
   return BF_OK;
 }
 

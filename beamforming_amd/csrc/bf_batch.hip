@@ -52,6 +52,7 @@ int check_channel_args(const bf_plan *p, int channel, int nfreq, int64_t ntimes,
   if (channel < 0 || (int64_t)channel + nfreq > p->shape.nfrequencies)
     return fail(BF_ERR_INVALID_ARGUMENT, "channels out of range");
   if (ntimes < 1) return fail(BF_ERR_INVALID_ARGUMENT, "ntimes must be >= 1");
+  if (ntimes > kMaxTimes) return fail(BF_ERR_UNSUPPORTED, "ntimes too large");
   return BF_OK;
 }
 
@@ -161,7 +162,7 @@ int upload(bf_batch *b, const bf_chunk *chunks, int n, hipStream_t st) {
       return fail(BF_ERR_INVALID_ARGUMENT, "chunk " + std::to_string(i) + ": E must be 16-byte and J 8-byte aligned");
   }
   for (int i = 0; i < n; ++i)
-    if ((chunks[i].ntimes + kTileT - 1) / kTileT > (1ll << 30)) return fail(BF_ERR_UNSUPPORTED, "chunk too long");
+    if (chunks[i].ntimes > kMaxTimes) return fail(BF_ERR_UNSUPPORTED, "chunk too long");
   std::vector<BatchItem> items; // segments
   std::vector<int> work;        // work item starts
   int it = 0;

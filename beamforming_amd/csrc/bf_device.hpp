@@ -163,6 +163,7 @@ BF_HD void corrections(int sum_ar, int sum_ai, int *corr_re, int *corr_im) {
 constexpr int kWaves = 8;            // waves per workgroup (2 per SIMD)
 constexpr int kThreads = 64 * kWaves;
 constexpr int kTileT = 16;           // time samples per wave tile (x 2 polarizations = MFMA M)
+constexpr int64_t kMaxTimes = int64_t(1) << 34; // longest chunk: time tile indices stay below 2^30
 constexpr int kDishesPerStep = 16;   // dishes per K step (MFMA K = 32 int8)
 constexpr int kBeamTile = 32;        // beams per MFMA tile (MFMA N)
 

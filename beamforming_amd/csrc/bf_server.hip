@@ -217,6 +217,7 @@ void run_loop(bf_server *s) {
       s->max_batch_seen = std::max<uint64_t>(s->max_batch_seen, L.tickets.size());
       lk.unlock();
       const int rc = issue(s, L, batch, chunks, copies); // Request pointers stay valid: entries are only erased on release
+      if (rc) (void)hipStreamSynchronize(L.stream); // no staging copy of a failed launch may still write the arena
       lk.lock();
       if (rc) {
         s->loop_error = g_last_error;
@@ -350,7 +351,8 @@ int bf_server_submit_many(bf_server *s, const bf_request *reqs, int n, uint64_t 
   for (int i = 0; i < n; ++i) { // validate everything before queueing anything
     const bf_request *req = &reqs[i];
     const int nf = req->nfreq == 0 ? 1 : req->nfreq;
-    if (!req->E || req->ntimes < 1 || nf < 1 || nf > 65535 || req->channel < 0 ||
+    // ntimes <= kMaxTimes (the batch kernel's limit) keeps the byte counts below in int64
+    if (!req->E || req->ntimes < 1 || req->ntimes > kMaxTimes || nf < 1 || nf > 65535 || req->channel < 0 ||
         (int64_t)req->channel + nf > sh.nfrequencies || req->reserved != 0 ||
         (req->e_on_host != 0 && req->e_on_host != 1))
       return fail(BF_ERR_INVALID_ARGUMENT, "bad request " + std::to_string(i) +

@@ -71,6 +71,23 @@ def test_invalid_shapes_rejected_before_any_device_call(bad):
     assert bf.lib.bf_last_error()
 
 
+@pytest.mark.parametrize("extents", [
+    (2**34 + 1, 1, 32, 16),      # longer than the time tile indices allow
+    (2**30, 2**20, 65536, 16),   # E bytes beyond 2^62
+    (2**33, 2**20, 32, 2**20),   # J bytes beyond 2^62
+])
+def test_oversized_shapes_rejected_before_any_device_call(extents):
+    shape = bf.Shape.make(*extents)
+    A = np.zeros(16, np.uint8)
+    G = np.zeros(16, np.float32)
+    h = ctypes.c_void_p()
+    rc = bf.lib.bf_plan_create(ctypes.byref(shape), A.ctypes.data_as(ctypes.c_void_p),
+                               G.ctypes.data_as(ctypes.c_void_p), ctypes.byref(h))
+    assert rc == 2  # BF_ERR_UNSUPPORTED
+    assert not h.value
+    assert b"too large" in bf.lib.bf_last_error()
+
+
 def test_null_arguments_rejected():
     shape = bf.Shape.make(16, 1, 32, 16)
     h = ctypes.c_void_p()
