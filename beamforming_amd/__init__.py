@@ -225,6 +225,7 @@ def _load():
         "bf_ingest_pop": (ctypes.c_int, [vp, vp, ctypes.POINTER(Chunk), ctypes.POINTER(ctypes.c_uint64)]),
         "bf_ingest_release": (ctypes.c_int, [vp, ctypes.c_uint64, vp]),
         "bf_ingest_pending": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)]),
+        "bf_ingest_sync": (ctypes.c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -543,7 +544,8 @@ class Ingest:
         self.close()
 
     def push(self, packet_ptr, ntimes, on_host):
-        """Appends a packet; returns False (nothing copied) when no ring slot is free."""
+        """Appends a packet; returns False (nothing copied) when no ring slot is free.
+        The packet is read asynchronously: keep it unchanged until sync()."""
         st = lib.bf_ingest_push(self._h, ctypes.c_void_p(packet_ptr), ntimes, int(bool(on_host)))
         if st == BF_ERR_NOT_READY:
             return False
@@ -562,6 +564,10 @@ class Ingest:
 
     def release(self, slot_id, stream=0):
         _check(lib.bf_ingest_release(self._h, slot_id, ctypes.c_void_p(stream)))
+
+    def sync(self):
+        """Waits until every pushed packet has been read."""
+        _check(lib.bf_ingest_sync(self._h))
 
     def pending(self):
         n = ctypes.c_int64()

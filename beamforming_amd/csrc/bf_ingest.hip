@@ -284,6 +284,12 @@ int bf_ingest_release(bf_ingest *g, uint64_t slot_id, void *stream) {
   return BF_OK;
 }
 
+int bf_ingest_sync(bf_ingest *g) {
+  if (!g) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  BF_HIP(hipStreamSynchronize(g->stream));
+  return BF_OK;
+}
+
 int bf_ingest_pending(const bf_ingest *g, int64_t *filled_samples, int *ready_chunks) {
   if (!g) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
   if (filled_samples) *filled_samples = g->fill_pos;

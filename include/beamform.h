@@ -343,8 +343,16 @@ int bf_ingest_destroy(bf_ingest *ingest);
 /* Appends a packet of ntimes samples (E[ntimes][F][D][P]; host memory if
  * packet_on_host, best pinned, else device memory). Packets continue each
  * other in time. Returns BF_ERR_NOT_READY, copying nothing, when a chunk it
- * would start has no free ring slot (the consumer has not released one). */
+ * would start has no free ring slot (the consumer has not released one).
+ * The packet is read asynchronously, on the ingest's own stream: keep it
+ * unchanged until bf_ingest_sync returns (or until a stream made to wait by
+ * bf_ingest_pop for a chunk holding the packet's last sample has passed
+ * that point). */
 int bf_ingest_push(bf_ingest *ingest, const uint8_t *packet, int64_t ntimes, int packet_on_host);
+
+/* Waits until every packet pushed so far has been read; the caller may then
+ * reuse the packet buffers. */
+int bf_ingest_sync(bf_ingest *ingest);
 
 /* Next complete chunk, in completion order (for every group in turn), or
  * BF_ERR_NOT_READY. `stream` is made to wait for the chunk's copies. */

@@ -48,7 +48,7 @@ def test_ingest_chunks_are_packet_slices(mode):
             if on_host:
                 pkt = np.ascontiguousarray(E[t:t + n]).reshape(-1)
                 assert ing.push(pkt.ctypes.data, n, True)
-                torch.cuda.synchronize()
+                ing.sync()  # the packet buffer may be reused from here on
             else:
                 assert ing.push(base + t * F * D * 2, n, False)
             t += n
