@@ -297,6 +297,7 @@ int alloc_weight_tables(bf_plan *p) {
   p->W = nullptr;
   p->corr = nullptr;
   p->Gp = nullptr;
+  ++p->layout_version; // batches re-derive their launch from the plan
   const size_t wbytes = (size_t)s.nfrequencies * c.ks * c.ntp * 64 * sizeof(v4i);
   const size_t BP = (size_t)c.ntp * kBeamTile;
   BF_HIP(hipMalloc(&p->W, wbytes));
@@ -393,8 +394,10 @@ int bf_plan_update_weights(bf_plan *p, const uint8_t *A, const float *G) {
     // The kernel family changes; rebuild the plan in place.
     bf_plan *q = nullptr;
     if (int rc = plan_create(&s, A, G, true, nullptr, &q)) return rc;
+    const uint64_t version = p->layout_version;
     std::swap(*p, *q);
     free_plan(q);
+    p->layout_version = version + 1; // new tables and possibly another kernel family
     return BF_OK;
   }
   BF_HIP(hipMemcpy(p->A_dev, A, s.nfrequencies * s.nbeams * s.ndishes, hipMemcpyHostToDevice));

@@ -42,6 +42,7 @@ struct bf_batch {
   hipStream_t capture_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  uint64_t plan_version = 0; // the plan's layout_version the grid and the graph were made for
 };
 
 namespace {
@@ -144,10 +145,37 @@ int64_t balanced_items(const bf_chunk *chunks, int n, int nbg, int grid, std::ve
   return share;
 }
 
+// Launch geometry for the plan's current configuration: the batch kernel's
+// LDS limit (the instantiation's largest layout, so plans of either layout
+// can share it) and one wave of resident workgroups. A captured graph of an
+// earlier configuration is dropped (it holds that configuration's kernel and
+// weight table addresses).
+int follow_plan(bf_batch *b) {
+  const bf_plan *p = b->plan;
+  if (b->grid > 0 && b->plan_version == p->layout_version) return BF_OK;
+  if (!p->use_mfma)
+    return fail(BF_ERR_UNSUPPORTED, "batching needs the MFMA path (P = 2, D % 8 == 0, D <= 1024, gains in range)");
+  const MfmaConfig &c = p->cfg;
+  if (c.nbg > 65535) return fail(BF_ERR_UNSUPPORTED, "too many beam groups for a batch work item");
+  if (b->launch_recorded) BF_HIP(hipEventSynchronize(b->launch_done));
+  free_graph(b);
+  BF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(c.batch_kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)mfma_lds_bytes(c.nt, c.ks)));
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(c.batch_kernel), kThreads,
+                                                   c.lds_bytes) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  b->grid = p->num_cus * per_cu;
+  b->plan_version = p->layout_version;
+  return BF_OK;
+}
+
 // Validates the chunk list, builds the work items (longest first) into the
 // pinned staging buffers and copies both to the device on `st`.
 int upload(bf_batch *b, const bf_chunk *chunks, int n, hipStream_t st) {
   const bf_plan *p = b->plan;
+  if (int rc = follow_plan(b)) return rc;
   if (n < 0 || n > b->max_chunks) return fail(BF_ERR_INVALID_ARGUMENT, "nchunks outside [0, max_chunks]");
   if (n > 0 && !chunks) return fail(BF_ERR_INVALID_ARGUMENT, "chunks is NULL");
   const MfmaConfig &c = p->cfg;
@@ -291,9 +319,6 @@ int bf_batch_create(const bf_plan *p, int max_chunks, bf_batch **out) {
   if (!out) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
   *out = nullptr;
   if (!p || max_chunks < 1) return fail(BF_ERR_INVALID_ARGUMENT, "NULL plan or max_chunks < 1");
-  if (!p->use_mfma)
-    return fail(BF_ERR_UNSUPPORTED, "batching needs the MFMA path (P = 2, D % 8 == 0, D <= 1024, gains in range)");
-  if (p->cfg.nbg > 65535) return fail(BF_ERR_UNSUPPORTED, "too many beam groups for a batch work item");
   bf_batch *b = new bf_batch();
   b->plan = p;
   b->max_chunks = max_chunks;
@@ -301,19 +326,8 @@ int bf_batch_create(const bf_plan *p, int max_chunks, bf_batch **out) {
     bf_batch_destroy(b);
     return rc;
   };
-  const MfmaConfig &c = p->cfg;
+  if (int rc = follow_plan(b)) return bail(rc);
   hipError_t e;
-  // the instantiation's largest layout, so plans of either layout can share it
-  if ((e = hipFuncSetAttribute(reinterpret_cast<const void *>(c.batch_kernel),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)mfma_lds_bytes(c.nt, c.ks))) !=
-      hipSuccess)
-    return bail(hip_fail(e, "hipFuncSetAttribute(batch)"));
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(c.batch_kernel), kThreads,
-                                                   c.lds_bytes) != hipSuccess ||
-      per_cu < 1)
-    per_cu = 1;
-  b->grid = p->num_cus * per_cu;
   if ((e = hipEventCreateWithFlags(&b->upload_done, hipEventDisableTiming)) != hipSuccess)
     return bail(hip_fail(e, "hipEventCreate"));
   if ((e = hipEventCreateWithFlags(&b->launch_done, hipEventDisableTiming)) != hipSuccess)

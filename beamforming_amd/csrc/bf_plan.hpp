@@ -43,6 +43,7 @@ struct bf_plan {
   bool fast_gains = false; // all gains inside the fast requantisation range
   bool use_mfma = false;   // MFMA kernel (else the generic kernel)
   bf::MfmaConfig cfg;
+  uint64_t layout_version = 0; // bumped whenever cfg or the weight tables' addresses change
   int wgs_per_cu = 1;      // resident MFMA workgroups per CU
   uint8_t *A_dev = nullptr; // [F][B][D] as given
   float *G_dev = nullptr;   // [F][B] as given

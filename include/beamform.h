@@ -162,7 +162,11 @@ typedef struct bf_batch bf_batch;
 
 /* Creates a batch context for up to max_chunks chunks of a channel-table
  * plan: device storage for the chunk descriptors and the work queue, and a
- * hipGraph of the launch, captured on first use of bf_batch_run_graph. */
+ * hipGraph of the launch, captured on first use of bf_batch_run_graph. The
+ * batch follows its plan: after bf_plan_configure, bf_plan_autotune or
+ * bf_plan_update_weights the next run re-derives its launch (and captures the
+ * graph again); it fails with BF_ERR_UNSUPPORTED while new gains keep the
+ * plan off the MFMA kernel. */
 int bf_batch_create(const bf_plan *channel_plan, int max_chunks, bf_batch **batch);
 int bf_batch_destroy(bf_batch *batch);
 
@@ -190,13 +194,6 @@ int bf_batch_run_graph(bf_batch *batch, const bf_chunk *chunks, int nchunks, voi
 /* One-line description of the batch configuration. */
 int bf_batch_describe(const bf_batch *batch, char *buf, size_t len);
 
-/* Times candidate kernel configurations (beam tiles per workgroup, LDS
- * layout: J rows staged in LDS for whole-line stores or stored directly, and
- * the number of workgroups, i.e. the time split) on the caller's device
- * buffers E and J with the plan's shape, keeps the fastest in the plan, and
- * remembers it for later plans of the same shape in this process. `iters`
- * timed launches per candidate (0 = default). A report of the candidates is
- * written to report (may be NULL). Synchronous. */
 /* Sets the kernel configuration explicitly, as the autotuner does after
  * timing: beam_tiles 32-beam tiles per workgroup (1..4, 0 = automatic),
  * stage_j 1 to stage J rows in LDS where the layout and the LDS allow it,
@@ -204,6 +201,13 @@ int bf_batch_describe(const bf_batch *batch, char *buf, size_t len);
  * with the plan must have finished. Synchronous. */
 int bf_plan_configure(bf_plan *plan, int beam_tiles, int stage_j, double wg_scale);
 
+/* Times candidate kernel configurations (beam tiles per workgroup, LDS
+ * layout: J rows staged in LDS for whole-line stores or stored directly, and
+ * the number of workgroups, i.e. the time split) on the caller's device
+ * buffers E and J with the plan's shape, keeps the fastest in the plan, and
+ * remembers it for later plans of the same shape in this process. `iters`
+ * timed launches per candidate (0 = default). A report of the candidates is
+ * written to report (may be NULL). Synchronous. */
 int bf_plan_autotune(bf_plan *plan, uint8_t *J, const uint8_t *E, void *stream, int iters, char *report,
                      size_t report_len);
 

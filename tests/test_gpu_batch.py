@@ -229,6 +229,45 @@ def test_configured_layouts_are_exact(nt, stage):
     check_chunks(specs, host, dev, A, G, D, B)
 
 
+def test_batch_follows_plan_reconfiguration():
+    # A batch made before its plan is reconfigured (other beam tiling, LDS
+    # layout, workgroup count, new weight tables) or given new weights keeps
+    # giving exact results, through the captured graph too; gains that move
+    # the plan off the MFMA path make batch runs fail cleanly.
+    C, D, B = 4, 512, 96
+    A, G = channel_table(C, D, B, 31)
+    specs = [(900, 0), (333, 3), (2048, 1), (77, 2)]
+    host, dev = make_chunks(specs, D, B, 700)
+
+    def run_and_check(batch, graph, A, G):
+        for _, dJ in dev:
+            dJ.zero_()
+        torch.cuda.synchronize()
+        batch.run(chunk_list(specs, dev), stream(), graph=graph)
+        torch.cuda.synchronize()
+        say(f"  {batch.describe()}")
+        check_chunks(specs, host, dev, A, G, D, B)
+
+    with bf.Plan(bf.Shape.make(1, C, D, B), A, G) as table, bf.Batch(table, 8) as batch:
+        run_and_check(batch, True, A, G)
+        for nt, stage, scale in [(1, False, 1.0), (4, False, 2.0), (3, True, 0.5)]:
+            table.configure(nt, stage, scale)
+            say(table.describe())
+            run_and_check(batch, True, A, G)
+            run_and_check(batch, False, A, G)
+        A2, G2 = channel_table(C, D, B, 32)
+        table.update_weights(A2, G2)
+        run_and_check(batch, True, A2, G2)
+        Gbig = np.full_like(G, 1e30)  # outside the fast requantisation range: generic kernel
+        table.update_weights(A2, Gbig)
+        with pytest.raises(bf.BeamformError) as ei:
+            batch.run(chunk_list(specs, dev), stream())
+        assert ei.value.status == bf.BF_ERR_UNSUPPORTED
+        torch.cuda.synchronize()
+        table.update_weights(A, G)
+        run_and_check(batch, True, A, G)
+
+
 @pytest.mark.parametrize("layout", [bf.BF_J_BFPT, bf.BF_J_BFTP])
 def test_multi_channel_plain_and_batched(layout):
     C, D, B = 20, 512, 96
