@@ -436,25 +436,78 @@ int bf_form_beams_device(const bf_plan *p, uint8_t *J, const uint8_t *E, void *s
   return launch_form_beams(p, J, E, reinterpret_cast<hipStream_t>(stream));
 }
 
+// Host buffers: the time axis is cut into slabs that go through two device
+// slab buffers, with the copies in, the kernels and the copies out on three
+// streams, so that E streams in back to back (the PCIe-bound part) while
+// earlier slabs are beamformed and their J goes out (full duplex). A slab's
+// J is [b][f][p][n] (or [b][f][n][p]); one 2-D copy puts its rows into the
+// columns t0 .. t0 + n - 1 of the caller's J.
 int bf_form_beams_host(const bf_plan *p, uint8_t *J, const uint8_t *E) {
   if (!p || !J || !E) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
   const bf_shape &s = p->shape;
-  const size_t nE = (size_t)(s.ntimes * s.nfrequencies * s.ndishes * s.npolarizations);
-  const size_t nJ = (size_t)(s.nbeams * s.nfrequencies * s.npolarizations * s.ntimes);
-  uint8_t *dE = nullptr, *dJ = nullptr;
-  hipError_t e = hipMalloc(&dE, nE);
-  if (e != hipSuccess) return hip_fail(e, "hipMalloc E");
-  e = hipMalloc(&dJ, nJ);
+  const int64_t T = s.ntimes;
+  const size_t e_row = (size_t)(s.nfrequencies * s.ndishes * s.npolarizations); // E bytes per time sample
+  const size_t j_row = (size_t)(s.nbeams * s.nfrequencies * s.npolarizations);  // J bytes per time sample
+  // about 8 slabs, whole 16-sample tiles, at least 256 samples
+  const int64_t slab = T <= 256 ? T : std::max<int64_t>(256, ((T + 7) / 8 + 15) / 16 * 16);
+  const int64_t nslab = (T + slab - 1) / slab;
+  const int nbuf = nslab > 1 ? 2 : 1;
+  enum { IN, RUN, OUT };
+  hipStream_t st[3] = {nullptr, nullptr, nullptr};
+  // per slab buffer: copy in done, kernel done (E buffer free), copy out done (J buffer free)
+  hipEvent_t ev_in[2] = {nullptr, nullptr}, ev_run[2] = {nullptr, nullptr}, ev_out[2] = {nullptr, nullptr};
+  uint8_t *dE[2] = {nullptr, nullptr}, *dJ[2] = {nullptr, nullptr};
+  uint8_t *slabs = nullptr; // one allocation: E and J slab buffers, 256-byte aligned
+  const size_t e_bytes = ((size_t)slab * e_row + 255) / 256 * 256, j_bytes = ((size_t)slab * j_row + 255) / 256 * 256;
   int rc = BF_OK;
-  if (e != hipSuccess) {
-    rc = hip_fail(e, "hipMalloc J");
-  } else if ((e = hipMemcpy(dE, E, nE, hipMemcpyHostToDevice)) != hipSuccess) {
-    rc = hip_fail(e, "copy E");
-  } else if ((rc = launch_form_beams(p, dJ, dE, nullptr)) == BF_OK) {
-    if ((e = hipMemcpy(J, dJ, nJ, hipMemcpyDeviceToHost)) != hipSuccess) rc = hip_fail(e, "copy J");
+  hipError_t e = hipSuccess;
+  auto check = [&](hipError_t err, const char *what) {
+    if (err != hipSuccess && rc == BF_OK) rc = hip_fail(err, what);
+    return rc == BF_OK;
+  };
+  for (int k = 0; k < 3 && rc == BF_OK; ++k) check(hipStreamCreateWithFlags(&st[k], hipStreamNonBlocking), "hipStreamCreate");
+  for (int k = 0; k < nbuf && rc == BF_OK; ++k)
+    check(hipEventCreateWithFlags(&ev_in[k], hipEventDisableTiming), "hipEventCreate") &&
+        check(hipEventCreateWithFlags(&ev_run[k], hipEventDisableTiming), "hipEventCreate") &&
+        check(hipEventCreateWithFlags(&ev_out[k], hipEventDisableTiming), "hipEventCreate");
+  if (rc == BF_OK && check(hipMalloc(&slabs, nbuf * (e_bytes + j_bytes)), "hipMalloc slabs"))
+    for (int k = 0; k < nbuf; ++k) {
+      dE[k] = slabs + k * (e_bytes + j_bytes);
+      dJ[k] = dE[k] + e_bytes;
+    }
+  const bool tp = s.j_layout == BF_J_BFTP; // J rows of n samples x P, else of n samples
+  const size_t pitch = tp ? (size_t)T * s.npolarizations : (size_t)T;
+  const size_t rows = tp ? (size_t)(s.nbeams * s.nfrequencies) : (size_t)(s.nbeams * s.nfrequencies * s.npolarizations);
+  for (int64_t i = 0; i < nslab && rc == BF_OK; ++i) {
+    const int k = (int)(i % nbuf);
+    const bool reuse = i >= nbuf; // slab i - 2 used these buffers
+    const int64_t t0 = i * slab, n = std::min(slab, T - t0);
+    if (reuse && !check(hipStreamWaitEvent(st[IN], ev_run[k], 0), "hipStreamWaitEvent")) break;
+    if (!check(hipMemcpyAsync(dE[k], E + (size_t)t0 * e_row, (size_t)n * e_row, hipMemcpyHostToDevice, st[IN]),
+               "copy E slab") ||
+        !check(hipEventRecord(ev_in[k], st[IN]), "hipEventRecord") ||
+        !check(hipStreamWaitEvent(st[RUN], ev_in[k], 0), "hipStreamWaitEvent") ||
+        (reuse && !check(hipStreamWaitEvent(st[RUN], ev_out[k], 0), "hipStreamWaitEvent")))
+      break;
+    if ((rc = bf_form_beams_channels(p, 0, (int)s.nfrequencies, n, dJ[k], dE[k], st[RUN])) != BF_OK) break;
+    const size_t width = tp ? (size_t)n * s.npolarizations : (size_t)n;
+    const size_t col = tp ? (size_t)t0 * s.npolarizations : (size_t)t0;
+    if (!check(hipEventRecord(ev_run[k], st[RUN]), "hipEventRecord") ||
+        !check(hipStreamWaitEvent(st[OUT], ev_run[k], 0), "hipStreamWaitEvent") ||
+        !check(hipMemcpy2DAsync(J + col, pitch, dJ[k], width, width, rows, hipMemcpyDeviceToHost, st[OUT]),
+               "copy J slab") ||
+        !check(hipEventRecord(ev_out[k], st[OUT]), "hipEventRecord"))
+      break;
   }
-  (void)hipFree(dE);
-  if (dJ) (void)hipFree(dJ);
+  for (int k = 0; k < 3; ++k)
+    if (st[k] && (e = hipStreamSynchronize(st[k])) != hipSuccess && rc == BF_OK) rc = hip_fail(e, "host path");
+  if (slabs) (void)hipFree(slabs);
+  for (int k = 0; k < nbuf; ++k) {
+    for (hipEvent_t ev : {ev_in[k], ev_run[k], ev_out[k]})
+      if (ev) (void)hipEventDestroy(ev);
+  }
+  for (int k = 0; k < 3; ++k)
+    if (st[k]) (void)hipStreamDestroy(st[k]);
   return rc;
 }
 

@@ -107,7 +107,10 @@ int bf_plan_describe(const bf_plan *plan, char *buf, size_t len);
 int bf_form_beams_device(const bf_plan *plan, uint8_t *J, const uint8_t *E, void *stream);
 
 /* form_beams on host buffers: copies E in, runs, copies J out, synchronously
- * (the flow of cuda.cu:274-314). */
+ * (the flow of cuda.cu:274-314). The time axis goes through the device in
+ * slabs, the copies in, the kernels and the copies out on three streams, so
+ * that they overlap (best with pinned host buffers); device memory use is
+ * two slabs of about T / 8 samples each, not whole E and J. */
 int bf_form_beams_host(const bf_plan *plan, uint8_t *J, const uint8_t *E);
 
 /* One-shot drop-in for the reference entry point with an explicit shape, all

@@ -268,6 +268,32 @@ def test_host_entry_points():
         assert np.array_equal(plan.form_beams_host(E), want)
 
 
+@pytest.mark.parametrize("T,F,D,B,layout,pinned", [
+    (1000, 3, 64, 40, bf.BF_J_BFPT, False),    # 4 slabs of 256 samples, the last 232
+    (4099, 2, 128, 96, bf.BF_J_BFTP, True),    # 8 slabs of 528, the last 403; pinned buffers
+    (700, 2, 12, 5, bf.BF_J_BFPT, False),      # generic kernel (D not a multiple of 8)
+    (700, 2, 12, 5, bf.BF_J_BFTP, True),
+    (255, 1, 32, 16, bf.BF_J_BFPT, False),     # one slab
+])
+def test_host_path_slabs(T, F, D, B, layout, pinned):
+    # the host entry point runs the time axis in slabs on two streams; every
+    # slab's J lands in its columns of the caller's J
+    say(f"host path T={T} F={F} D={D} B={B} layout={layout} pinned={pinned}")
+    shape = bf.Shape.make(T, F, D, B, j_layout=layout)
+    E, A, G = seeded(T, F, D, B, 61 + T)
+    want = bo.form_beams(E, A, G, T, F, D, B, jlayout=layout)
+    with bf.Plan(shape, A, G) as plan:
+        if pinned:
+            hE = torch.from_numpy(E).pin_memory()
+            hJ = torch.empty(shape.j_bytes, dtype=torch.uint8).pin_memory()
+            plan.form_beams_host_ptr(hJ.data_ptr(), hE.data_ptr())
+            got = hJ.numpy()
+        else:
+            got = plan.form_beams_host(E)
+    assert np.array_equal(got, want)
+    assert np.array_equal(bf.form_beams(E, A, G, shape), want)
+
+
 def test_update_weights_switches_kernels():
     T, F, D, B = 128, 2, 64, 32
     shape = bf.Shape.make(T, F, D, B)
