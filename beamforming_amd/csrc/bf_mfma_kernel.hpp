@@ -30,9 +30,8 @@
 //    stores its 16 bytes directly.
 // Rows past T and dishes past D are clamped, always-in-bounds loads: the
 // outputs of rows past T are never stored and the weights of dishes past D
-// are zero. The plan
-// routes gains outside the fast requantisation range (non-finite, or
-// |G * raw| >= 2^30 possible) to the generic kernel.
+// are zero. The plan routes gains outside the fast requantisation range
+// (non-finite, or |G * raw| >= 2^30 possible) to the generic kernel.
 #pragma once
 
 #include "bf_device.hpp"
