@@ -111,14 +111,15 @@ struct bf_server {
 
 namespace {
 
-// Issues one launch: staging copies of host-fed E, then the batched kernel,
-// then the completion event. Called without the server lock.
+// The weight set of generation gen (nullptr once it is freed).
 WeightSet *set_of(bf_server *s, uint64_t gen) {
   for (WeightSet &w : s->sets)
     if (w.gen == gen) return &w;
   return nullptr;
 }
 
+// Issues one launch: staging copies of host-fed E, then the batched kernel,
+// then the completion event. Called without the server lock.
 int issue(bf_server *s, Launch &L, bf_batch *batch, const std::vector<bf_chunk> &chunks,
           const std::vector<std::pair<uint8_t *, const Request *>> &copies) {
   for (const auto &c : copies) {
