@@ -103,7 +103,7 @@ class ServerConfig(ctypes.Structure):
         ("max_batch", ctypes.c_int32),
         ("max_launches", ctypes.c_int32),
         ("use_graph", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("poll_us", ctypes.c_int32),
     ]
 
 
@@ -432,9 +432,9 @@ class Server:
     """The streaming request loop over a channel-table plan (continuous
     batching with HBM-budget admission)."""
 
-    def __init__(self, plan, hbm_budget, max_inflight=0, max_batch=0, max_launches=0, use_graph=False):
+    def __init__(self, plan, hbm_budget, max_inflight=0, max_batch=0, max_launches=0, use_graph=False, poll_us=0):
         self.plan = plan  # the plan must outlive the server
-        cfg = ServerConfig(hbm_budget, max_inflight, max_batch, max_launches, int(bool(use_graph)), 0)
+        cfg = ServerConfig(hbm_budget, max_inflight, max_batch, max_launches, int(bool(use_graph)), int(poll_us))
         self._h = ctypes.c_void_p()
         _check(lib.bf_server_create(plan.handle, ctypes.byref(cfg), ctypes.byref(self._h)))
 

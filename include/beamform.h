@@ -228,7 +228,8 @@ int bf_batch_autotune(bf_batch *batch, const bf_chunk *chunks, int nchunks, void
  * them in arrival order while a slot is free and the HBM budget (one device
  * arena allocated at creation) has room for the request's J and, for host-fed
  * E, its staging copy; it launches everything admitted as one persistent
- * batched launch (bf_batch_run) on one of max_launches streams, so requests
+ * batched launch (bf_batch_run; a lone request as bf_form_beams_channels,
+ * which needs no descriptor copy) on one of max_launches streams, so requests
  * arriving while others run are taken into the next launch and overlap the
  * tail of the earlier one. J stays in the arena until bf_server_release.
  * The plan must outlive the server.
@@ -241,8 +242,11 @@ typedef struct bf_server_config {
   int32_t max_inflight; /* requests admitted and not finished at once (0 = 256) */
   int32_t max_batch;    /* requests per launch (0 = 128) */
   int32_t max_launches; /* launches in flight at once, one stream each (0 = 2) */
-  int32_t use_graph;    /* 1: each launch is a replay of a captured hipGraph */
-  int32_t reserved;     /* must be 0 */
+  int32_t use_graph;    /* 1: each launch of several requests is a replay of a captured hipGraph */
+  int32_t poll_us;      /* while launches are in flight, the loop thread polls their completion:
+                           0 (default) without sleeping, yielding its core (lowest latency; one core
+                           stays busy while the GPU works); N in 1..1000000 sleeps up to N us between
+                           polls (the OS may wake it tens of microseconds late) */
 } bf_server_config;
 
 typedef struct bf_request {

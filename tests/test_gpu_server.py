@@ -66,11 +66,13 @@ def collect(server, n, timeout=30.0):
     return got
 
 
-def test_server_results_match_oracle(table):
+@pytest.mark.parametrize("poll_us", [0, 50])
+def test_server_results_match_oracle(table, poll_us):
+    # poll_us 0: the loop polls launches in flight without sleeping; 50: timed waits
     A, G, plan = table
     reqs = random_reqs(24, 1)
     torch.cuda.synchronize()
-    with bf.Server(plan, hbm_budget=1 << 30) as srv:
+    with bf.Server(plan, hbm_budget=1 << 30, poll_us=poll_us) as srv:
         for i, q in enumerate(reqs):
             srv.submit(q.dE.data_ptr(), q.T, q.channel, q.nf, tag=i)
         got = collect(srv, len(reqs))
@@ -156,6 +158,9 @@ def test_bad_requests_and_shutdown(table):
     torch.cuda.synchronize()
     with pytest.raises(bf.BeamformError):
         bf.Server(plan, hbm_budget=0)
+    for poll_us in (-1, 1000001):
+        with pytest.raises(bf.BeamformError):
+            bf.Server(plan, hbm_budget=1 << 20, poll_us=poll_us)
     srv = bf.Server(plan, hbm_budget=1 << 26)
     for bad in [dict(ntimes=0, channel=0, nfreq=1), dict(ntimes=64, channel=C, nfreq=1),
                 dict(ntimes=64, channel=C - 2, nfreq=4), dict(ntimes=64, channel=-1, nfreq=1)]:
