@@ -297,9 +297,8 @@ int bf_form_beams_channels(const bf_plan *p, int channel, int nfreq, int64_t nti
   const bf_shape &s = p->shape;
   const bool aligned = ((uintptr_t)E & 15u) == 0 && ((uintptr_t)J & 7u) == 0;
   if (p->use_mfma && aligned) {
-    MfmaConfig c = p->cfg;
-    if (!choose_grid(ntimes, nfreq, (int64_t)p->num_cus * p->wgs_per_cu, &c))
-      return fail(BF_ERR_UNSUPPORTED, "ntimes too large");
+    MfmaConfig c;
+    if (!pick_config(p, ntimes, nfreq, &c)) return fail(BF_ERR_UNSUPPORTED, "ntimes too large");
     return launch_mfma(p, c, J, E, ntimes, nfreq, channel, st);
   }
   const int64_t total = s.nbeams * nfreq * s.npolarizations * ntimes;
@@ -410,6 +409,7 @@ int bf_plan_autotune(bf_plan *p, uint8_t *J, const uint8_t *E, void *stream, int
   if (iters <= 0) iters = 10;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const bf_shape s = p->shape;
+  p->auto_config = false; // the tuned configuration is used as it is, for every launch
   std::string rep;
   TuneChoice best;
   float best_ms = 1e30f;
@@ -435,7 +435,7 @@ int bf_plan_autotune(bf_plan *p, uint8_t *J, const uint8_t *E, void *stream, int
         if (int rc = alloc_weight_tables(p)) return rc;
         if (int rc = prepare_weights(p, st)) return rc;
         float ms = 0;
-        if (int rc = time_launches(st, iters, &ms, [&] { return launch_form_beams(p, J, E, st); })) return rc;
+        if (int rc = time_launches(st, iters, &ms, [&] { return launch_form_beams(p, J, E, st, false); })) return rc;
         char line[160];
         std::snprintf(line, sizeof line, "nt=%d nbg=%d stage_j=%d scale=%.1f grid=%d lds=%zu: %.1f us\n", c.nt, c.nbg,
                       c.stage_j ? 1 : 0, scale, c.grid, c.lds_bytes, ms * 1e3f);
@@ -470,6 +470,7 @@ int bf_plan_configure(bf_plan *p, int beam_tiles, int stage_j, double wg_scale) 
   MfmaConfig c;
   if (!choose_mfma_config(p->shape, p->num_cus, &c, &choice))
     return fail(BF_ERR_UNSUPPORTED, "no MFMA configuration with these beam tiles for the shape");
+  p->auto_config = false; // launches use exactly this configuration
   p->cfg = c;
   p->wgs_per_cu = wgs_per_cu(c);
   if (int rc = alloc_weight_tables(p)) return rc;

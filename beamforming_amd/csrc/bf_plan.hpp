@@ -43,6 +43,13 @@ struct bf_plan {
   bool fast_gains = false; // all gains inside the fast requantisation range
   bool use_mfma = false;   // MFMA kernel (else the generic kernel)
   bf::MfmaConfig cfg;
+  // Small launches (fewer workgroups than CUs with cfg) take one beam tile
+  // per workgroup instead: NT times the workgroups over the same weight
+  // tables. Only for the automatic configuration, not a configured or tuned one.
+  bool auto_config = false;
+  bool has_narrow = false;
+  bf::MfmaConfig cfg_narrow;
+  int narrow_wgs_per_cu = 1;
   uint64_t layout_version = 0; // bumped whenever cfg or the weight tables' addresses change
   int wgs_per_cu = 1;      // resident MFMA workgroups per CU
   uint8_t *A_dev = nullptr; // [F][B][D] as given
@@ -68,5 +75,6 @@ int alloc_weight_tables(bf_plan *p);
 int prepare_weights(bf_plan *p, hipStream_t st);
 int launch_mfma(const bf_plan *p, const MfmaConfig &c, uint8_t *J, const uint8_t *E, int64_t T, int F, int wrow0,
                 hipStream_t st);
-int launch_form_beams(const bf_plan *p, uint8_t *J, const uint8_t *E, hipStream_t st);
+int launch_form_beams(const bf_plan *p, uint8_t *J, const uint8_t *E, hipStream_t st, bool allow_narrow = true);
+bool pick_config(const bf_plan *p, int64_t T, int64_t F, MfmaConfig *c);
 } // namespace bf

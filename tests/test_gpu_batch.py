@@ -290,3 +290,43 @@ def test_multi_channel_plain_and_batched(layout):
         want = bo.form_beams(E, A.reshape(C, -1)[c:c + nf], G.reshape(C, -1)[c:c + nf], T, nf, D, B, jlayout=layout)
         assert np.array_equal(dJp[i].cpu().numpy(), want), f"plain launch, chunk {i}"
         assert np.array_equal(dJb[i].cpu().numpy(), want), f"batched, chunk {i}"
+
+
+@pytest.mark.parametrize("D,B,layout", [(512, 96, bf.BF_J_BFPT), (512, 96, bf.BF_J_BFTP), (256, 192, bf.BF_J_BFPT),
+                                        (520, 128, bf.BF_J_BFPT)])
+def test_small_launches_with_one_beam_tile_per_workgroup(D, B, layout):
+    # launches that would leave CUs idle with the plan's beam tiling run one
+    # beam tile per workgroup over the same weight tables: exact as well
+    C = 6
+    A, G = channel_table(C, D, B, 41)
+    shape = bf.Shape.make(1, C, D, B, j_layout=layout)
+    specs = [(1024, 0), (16, 5), (3000, 2), (16384, 1), (1, 3)]
+    host, dev = make_chunks(specs, D, B, 900)
+    with bf.Plan(shape, A, G) as plan:
+        desc = plan.describe()
+        say(f"D={D} B={B} layout={layout}: {desc}")
+        assert "launches under one workgroup per CU: nt=1" in desc
+        for (T, c), (dE, dJ) in zip(specs, dev):
+            plan.form_beams_channel(c, T, dJ.data_ptr(), dE.data_ptr(), stream())
+        torch.cuda.synchronize()
+        check_chunks(specs, host, dev, A, G, D, B, layout)
+        # several channels in one launch
+        T, c0, nf = 700, 1, 4
+        E = bo.fill_uniform(T * nf * D * 2, 77)
+        dE = torch.from_numpy(E).cuda()
+        dJ = torch.zeros(B * nf * 2 * T, dtype=torch.uint8, device="cuda")
+        plan.form_beams_channels(c0, nf, T, dJ.data_ptr(), dE.data_ptr(), stream())
+        torch.cuda.synchronize()
+    Ac = A.reshape(C, B * D)[c0:c0 + nf].reshape(-1)
+    Gc = G.reshape(C, B)[c0:c0 + nf].reshape(-1)
+    assert np.array_equal(dJ.cpu().numpy(), bo.form_beams(E, Ac, Gc, T, nf, D, B, jlayout=layout))
+    # a whole small plan (its own shape) takes the same route
+    T, F = 300, 2
+    shape2 = bf.Shape.make(T, F, D, B, j_layout=layout)
+    E2 = bo.fill_uniform(T * F * D * 2, 78)
+    with bf.Plan(shape2, A[:F * B * D], G[:F * B]) as plan2:
+        dE2 = torch.from_numpy(E2).cuda()
+        dJ2 = torch.zeros(shape2.j_bytes, dtype=torch.uint8, device="cuda")
+        plan2.form_beams_device(dJ2.data_ptr(), dE2.data_ptr(), stream())
+        torch.cuda.synchronize()
+    assert np.array_equal(dJ2.cpu().numpy(), bo.form_beams(E2, A[:F * B * D], G[:F * B], T, F, D, B, jlayout=layout))
