@@ -1,9 +1,11 @@
-// bf_batch.hip -- one-channel launches, the batched persistent launch with
-// its device-side work queue and hipGraph replay, and the autotuners.
+// bf_batch.hip -- one-channel launches, the batched persistent launch (its
+// descriptors in pinned host memory) with hipGraph replay, and the
+// autotuners.
 #include "beamform.h"
 #include "bf_plan.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <functional>
 #include <string>
@@ -24,25 +26,29 @@ struct bf_batch {
   int item_tiles = 0; // 0 = automatic
   int last_item_tiles = 0;
   int last_nitems = 0;
+  bool last_queue = false; // the last prepared run takes its items from the queue
   int last_segments = 0;
   int grid = 0;
-  // One descriptor block, uploaded with a single copy per launch:
-  // [work item count, queue head (0) | pad to 16][chunks: max_chunks]
-  // [work item starts: item_capacity + 1, padded to 16][segments:
-  // item_capacity]. The copy also resets the queue head.
-  uint8_t *d_desc = nullptr;
-  uint8_t *h_desc = nullptr; // pinned staging
+  // Two descriptor blocks in pinned host memory, used by alternate runs and
+  // read by the kernel in place: [work item count | pad to 16][chunks:
+  // max_chunks][work item starts: item_capacity + 1, padded to 16][segments:
+  // item_capacity]. A run rewrites its block only after the launch that last
+  // read it (two runs before) has finished.
+  struct Block {
+    uint8_t *h = nullptr; // host address
+    uint8_t *d = nullptr; // the same memory as the kernel addresses it
+    unsigned *head = nullptr; // device: the queue counter of launches reading this block
+    uint32_t base = 0;        // its value when the next launch starts (it only grows)
+    hipEvent_t launch_done = nullptr;
+    bool launch_recorded = false;
+    hipGraph_t graph = nullptr; // captured launch reading this block
+    hipGraphExec_t exec = nullptr;
+  } blk[2];
+  int next_block = 0;
+  unsigned *d_heads = nullptr; // the two queue counters, 256 bytes apart
   size_t off_chunks = 16, off_work = 0, off_items = 0, desc_bytes = 0;
-  hipEvent_t upload_done = nullptr;
-  bool upload_pending = false;
-  // The last launch, so that the next descriptor copy (possibly on another
-  // stream) does not overwrite the block while that launch still reads it.
-  hipEvent_t launch_done = nullptr;
-  bool launch_recorded = false;
   hipStream_t capture_stream = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  uint64_t plan_version = 0; // the plan's layout_version the grid and the graph were made for
+  uint64_t plan_version = 0; // the plan's layout_version the grid and the graphs were made for
 };
 
 namespace {
@@ -59,36 +65,43 @@ int check_channel_args(const bf_plan *p, int channel, int nfreq, int64_t ntimes,
 
 int chunk_nfreq(const bf_chunk &k) { return k.nfreq == 0 ? 1 : k.nfreq; }
 
-void free_graph(bf_batch *b) {
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  b->exec = nullptr;
-  b->graph = nullptr;
+void free_graphs(bf_batch *b) {
+  for (bf_batch::Block &k : b->blk) {
+    if (k.exec) (void)hipGraphExecDestroy(k.exec);
+    if (k.graph) (void)hipGraphDestroy(k.graph);
+    k.exec = nullptr;
+    k.graph = nullptr;
+  }
+}
+
+// Waits until no launch reads either descriptor block.
+int wait_launches(bf_batch *b) {
+  for (bf_batch::Block &k : b->blk)
+    if (k.launch_recorded) BF_HIP(hipEventSynchronize(k.launch_done));
+  return BF_OK;
 }
 
 int ensure_item_capacity(bf_batch *b, int n) {
   if (n <= b->item_capacity) return BF_OK;
   int cap = std::max(n, 2 * b->item_capacity);
-  if (b->upload_pending) {
-    BF_HIP(hipEventSynchronize(b->upload_done));
-    b->upload_pending = false;
+  if (int rc = wait_launches(b)) return rc;
+  free_graphs(b); // the captured launches refer to the old blocks
+  for (bf_batch::Block &k : b->blk) {
+    if (k.h) (void)hipHostFree(k.h);
+    k.h = k.d = nullptr;
   }
-  // the previous block may still be read by a launch in flight
-  BF_HIP(hipDeviceSynchronize());
-  if (b->d_desc) (void)hipFree(b->d_desc);
-  if (b->h_desc) (void)hipHostFree(b->h_desc);
-  b->d_desc = nullptr;
-  b->h_desc = nullptr;
   b->item_capacity = 0;
   auto up16 = [](size_t x) { return (x + 15) / 16 * 16; };
   b->off_chunks = 16;
   b->off_work = b->off_chunks + up16((size_t)b->max_chunks * sizeof(BatchChunk));
   b->off_items = b->off_work + up16((size_t)(cap + 1) * sizeof(int));
   b->desc_bytes = b->off_items + (size_t)cap * sizeof(BatchItem);
-  BF_HIP(hipMalloc(&b->d_desc, b->desc_bytes));
-  BF_HIP(hipHostMalloc(&b->h_desc, b->desc_bytes, hipHostMallocDefault));
+  for (bf_batch::Block &k : b->blk) {
+    // coherent: the device reads what the host wrote before the launch, uncached
+    BF_HIP(hipHostMalloc(&k.h, b->desc_bytes, hipHostMallocCoherent));
+    BF_HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&k.d), k.h, 0));
+  }
   b->item_capacity = cap;
-  free_graph(b); // the captured launch refers to the old item buffer
   return BF_OK;
 }
 
@@ -157,8 +170,8 @@ int follow_plan(bf_batch *b) {
     return fail(BF_ERR_UNSUPPORTED, "batching needs the MFMA path (P = 2, D % 8 == 0, D <= 1024, gains in range)");
   const MfmaConfig &c = p->cfg;
   if (c.nbg > 65535) return fail(BF_ERR_UNSUPPORTED, "too many beam groups for a batch work item");
-  if (b->launch_recorded) BF_HIP(hipEventSynchronize(b->launch_done));
-  free_graph(b);
+  if (int rc = wait_launches(b)) return rc;
+  free_graphs(b);
   BF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(c.batch_kernel),
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)mfma_lds_bytes(c.nt, c.ks)));
   int per_cu = 0;
@@ -171,9 +184,9 @@ int follow_plan(bf_batch *b) {
   return BF_OK;
 }
 
-// Validates the chunk list, builds the work items (longest first) into the
-// pinned staging buffers and copies both to the device on `st`.
-int upload(bf_batch *b, const bf_chunk *chunks, int n, hipStream_t st) {
+// Validates the chunk list and writes it and the work items (longest first)
+// into the next descriptor block, whose index goes to *block.
+int prepare(bf_batch *b, const bf_chunk *chunks, int n, int *block) {
   const bf_plan *p = b->plan;
   if (int rc = follow_plan(b)) return rc;
   if (n < 0 || n > b->max_chunks) return fail(BF_ERR_INVALID_ARGUMENT, "nchunks outside [0, max_chunks]");
@@ -207,36 +220,36 @@ int upload(bf_batch *b, const bf_chunk *chunks, int n, hipStream_t st) {
   }
   const int nwork = (int)work.size() - 1;
   if (int rc = ensure_item_capacity(b, std::max<int>(1, (int)items.size()))) return rc;
-  if (b->upload_pending) { // the staging buffers may still be read by the previous copy
-    BF_HIP(hipEventSynchronize(b->upload_done));
-    b->upload_pending = false;
-  }
-  BatchChunk *hc = reinterpret_cast<BatchChunk *>(b->h_desc + b->off_chunks);
+  const int j = b->next_block;
+  bf_batch::Block &k = b->blk[j];
+  if (k.launch_recorded) BF_HIP(hipEventSynchronize(k.launch_done)); // the run two before still reads it
+  BatchChunk *hc = reinterpret_cast<BatchChunk *>(k.h + b->off_chunks);
   for (int i = 0; i < n; ++i)
     hc[i] = BatchChunk{chunks[i].E, chunks[i].J, chunks[i].ntimes, chunks[i].channel, chunk_nfreq(chunks[i])};
-  std::copy(items.begin(), items.end(), reinterpret_cast<BatchItem *>(b->h_desc + b->off_items));
-  std::copy(work.begin(), work.end(), reinterpret_cast<int *>(b->h_desc + b->off_work));
-  reinterpret_cast<int *>(b->h_desc)[0] = nwork;
-  reinterpret_cast<unsigned *>(b->h_desc)[1] = 0; // queue head
-  if (b->launch_recorded) BF_HIP(hipStreamWaitEvent(st, b->launch_done, 0));
-  BF_HIP(hipMemcpyAsync(b->d_desc, b->h_desc, b->off_items + items.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
-                        st));
-  BF_HIP(hipEventRecord(b->upload_done, st));
-  b->upload_pending = true;
+  std::copy(items.begin(), items.end(), reinterpret_cast<BatchItem *>(k.h + b->off_items));
+  std::copy(work.begin(), work.end(), reinterpret_cast<int *>(k.h + b->off_work));
+  reinterpret_cast<int *>(k.h)[0] = nwork;
+  reinterpret_cast<uint32_t *>(k.h)[1] = k.base;
+  reinterpret_cast<int *>(k.h)[2] = b->item_tiles > 0 ? 1 : 0; // balanced items: one per workgroup, no queue
+  b->last_queue = b->item_tiles > 0;
+  std::atomic_thread_fence(std::memory_order_seq_cst); // the block is written before the launch is enqueued
+  b->next_block = j ^ 1;
+  *block = j;
   b->last_item_tiles = it;
   b->last_nitems = nwork;
   b->last_segments = (int)items.size();
   return BF_OK;
 }
 
-BatchParams batch_params(const bf_batch *b) {
+BatchParams batch_params(const bf_batch *b, int j) {
   const bf_plan *p = b->plan;
+  const uint8_t *d = b->blk[j].d;
   BatchParams a;
-  a.chunks = reinterpret_cast<const BatchChunk *>(b->d_desc + b->off_chunks);
-  a.items = reinterpret_cast<const BatchItem *>(b->d_desc + b->off_items);
-  a.work_start = reinterpret_cast<const int *>(b->d_desc + b->off_work);
-  a.nitems = reinterpret_cast<const int *>(b->d_desc);
-  a.head = reinterpret_cast<unsigned *>(b->d_desc) + 1;
+  a.chunks = reinterpret_cast<const BatchChunk *>(d + b->off_chunks);
+  a.items = reinterpret_cast<const BatchItem *>(d + b->off_items);
+  a.work_start = reinterpret_cast<const int *>(d + b->off_work);
+  a.nitems = reinterpret_cast<const int *>(d);
+  a.head = b->blk[j].head;
   a.W = p->W;
   a.corr = p->corr;
   a.G = p->Gp;
@@ -248,20 +261,22 @@ BatchParams batch_params(const bf_batch *b) {
   return a;
 }
 
-// The launch itself: the persistent kernel (the descriptor copy before it
-// has reset the queue head).
-int enqueue_launch(bf_batch *b, hipStream_t st) {
+// The launch itself: the persistent kernel reading descriptor block j.
+int enqueue_launch(bf_batch *b, int j, hipStream_t st) {
   const MfmaConfig &c = b->plan->cfg;
-  BatchParams a = batch_params(b);
+  BatchParams a = batch_params(b, j);
   void *args[] = {&a};
   BF_HIP(hipLaunchKernel(reinterpret_cast<const void *>(c.batch_kernel), dim3(b->grid), dim3(kThreads), args,
                          c.lds_bytes, st));
   return BF_OK;
 }
 
-int record_launch(bf_batch *b, hipStream_t st) {
-  BF_HIP(hipEventRecord(b->launch_done, st));
-  b->launch_recorded = true;
+// After a launch on block j was enqueued: its queue counter will have grown
+// by the item count plus one failed fetch per workgroup.
+int record_launch(bf_batch *b, int j, hipStream_t st) {
+  if (b->last_queue) b->blk[j].base += (uint32_t)(b->last_nitems + b->grid);
+  BF_HIP(hipEventRecord(b->blk[j].launch_done, st));
+  b->blk[j].launch_recorded = true;
   return BF_OK;
 }
 
@@ -327,10 +342,13 @@ int bf_batch_create(const bf_plan *p, int max_chunks, bf_batch **out) {
   };
   if (int rc = follow_plan(b)) return bail(rc);
   hipError_t e;
-  if ((e = hipEventCreateWithFlags(&b->upload_done, hipEventDisableTiming)) != hipSuccess)
-    return bail(hip_fail(e, "hipEventCreate"));
-  if ((e = hipEventCreateWithFlags(&b->launch_done, hipEventDisableTiming)) != hipSuccess)
-    return bail(hip_fail(e, "hipEventCreate"));
+  for (bf_batch::Block &k : b->blk)
+    if ((e = hipEventCreateWithFlags(&k.launch_done, hipEventDisableTiming)) != hipSuccess)
+      return bail(hip_fail(e, "hipEventCreate"));
+  if ((e = hipMalloc(&b->d_heads, 512)) != hipSuccess) return bail(hip_fail(e, "hipMalloc (queue counters)"));
+  if ((e = hipMemset(b->d_heads, 0, 512)) != hipSuccess) return bail(hip_fail(e, "hipMemset (queue counters)"));
+  b->blk[0].head = b->d_heads;
+  b->blk[1].head = b->d_heads + 64;
   if ((e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking)) != hipSuccess)
     return bail(hip_fail(e, "hipStreamCreate"));
   if (int rc = ensure_item_capacity(b, 4 * b->grid)) return bail(rc);
@@ -340,14 +358,14 @@ int bf_batch_create(const bf_plan *p, int max_chunks, bf_batch **out) {
 
 int bf_batch_destroy(bf_batch *b) {
   if (!b) return BF_OK;
-  if (b->upload_pending) (void)hipEventSynchronize(b->upload_done);
-  if (b->launch_recorded) (void)hipEventSynchronize(b->launch_done); // the launch reads the descriptor block
-  free_graph(b);
+  (void)wait_launches(b); // the launches read the descriptor blocks
+  free_graphs(b);
   if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
-  if (b->upload_done) (void)hipEventDestroy(b->upload_done);
-  if (b->launch_done) (void)hipEventDestroy(b->launch_done);
-  if (b->d_desc) (void)hipFree(b->d_desc);
-  if (b->h_desc) (void)hipHostFree(b->h_desc);
+  for (bf_batch::Block &k : b->blk) {
+    if (k.launch_done) (void)hipEventDestroy(k.launch_done);
+    if (k.h) (void)hipHostFree(k.h);
+  }
+  if (b->d_heads) (void)hipFree(b->d_heads);
   delete b;
   return BF_OK;
 }
@@ -361,20 +379,23 @@ int bf_batch_set_item_tiles(bf_batch *b, int item_tiles) {
 int bf_batch_run(bf_batch *b, const bf_chunk *chunks, int nchunks, void *stream) {
   if (!b) return fail(BF_ERR_INVALID_ARGUMENT, "NULL batch");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  if (int rc = upload(b, chunks, nchunks, st)) return rc;
-  if (int rc = enqueue_launch(b, st)) return rc;
-  return record_launch(b, st);
+  int j = 0;
+  if (int rc = prepare(b, chunks, nchunks, &j)) return rc;
+  if (int rc = enqueue_launch(b, j, st)) return rc;
+  return record_launch(b, j, st);
 }
 
 int bf_batch_run_graph(bf_batch *b, const bf_chunk *chunks, int nchunks, void *stream) {
   if (!b) return fail(BF_ERR_INVALID_ARGUMENT, "NULL batch");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  if (int rc = upload(b, chunks, nchunks, st)) return rc;
-  if (!b->exec) {
-    // Capture once: the launch reads chunks, items and their count from the
-    // batch's device buffers, so the graph stays valid for any chunk list.
+  int j = 0;
+  if (int rc = prepare(b, chunks, nchunks, &j)) return rc;
+  bf_batch::Block &k = b->blk[j];
+  if (!k.exec) {
+    // Capture once per block: the launch reads chunks, items and their count
+    // from the block, so the graph stays valid for any chunk list.
     BF_HIP(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue_launch(b, b->capture_stream);
+    const int rc = enqueue_launch(b, j, b->capture_stream);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->capture_stream, &g);
     if (rc) {
@@ -382,11 +403,11 @@ int bf_batch_run_graph(bf_batch *b, const bf_chunk *chunks, int nchunks, void *s
       return rc;
     }
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-    b->graph = g;
-    BF_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    k.graph = g;
+    BF_HIP(hipGraphInstantiate(&k.exec, k.graph, nullptr, nullptr, 0));
   }
-  BF_HIP(hipGraphLaunch(b->exec, st));
-  return record_launch(b, st);
+  BF_HIP(hipGraphLaunch(k.exec, st));
+  return record_launch(b, j, st);
 }
 
 int bf_batch_describe(const bf_batch *b, char *buf, size_t len) {
@@ -398,7 +419,7 @@ int bf_batch_describe(const bf_batch *b, char *buf, size_t len) {
                 "segments=%d "
                 "graph=%s",
                 c.nt, c.ks, c.nbg, c.stage_j ? 1 : 0, b->grid, c.lds_bytes, b->last_item_tiles, b->item_tiles ? "" : " (balanced)",
-                b->last_nitems, b->last_segments, b->exec ? "captured" : "none");
+                b->last_nitems, b->last_segments, b->blk[0].exec || b->blk[1].exec ? "captured" : "none");
   return BF_OK;
 }
 

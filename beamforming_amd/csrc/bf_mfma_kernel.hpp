@@ -418,10 +418,16 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
 // Batched form_beams: independent chunks (nf consecutive channels each, own
 // T) in one persistent launch. The host cuts the work into segments (chunk,
 // frequency of the chunk, beam group, tile range) and groups consecutive
-// segments into work items; workgroups take items from a device-side queue
-// (an atomic head counter) until it is empty and run the item's segments in
-// order, restaging the weights in LDS only when the (channel, beam group)
-// changes.
+// segments into work items and runs an item's segments in order, restaging
+// the weights in LDS only when the (channel, beam group) changes. Balanced
+// items (at most one per workgroup) go to workgroup `item`; fixed-size items
+// come from a device-side queue. The queue is a counter in device memory
+// that only grows: a launch adds exactly its item count plus one failed fetch
+// per workgroup, so its items are the counter values base .. base + nitems - 1,
+// with base written by the host beside the item count, and nothing has to
+// reset it. The descriptors (item count, base and mode, item starts,
+// segments, chunks) are read straight from pinned host memory, so a launch
+// needs no copy before it.
 struct BatchChunk {
   const uint8_t *E; // [T][nf][D][2]
   uint8_t *J;       // [B][nf][2][T] or [B][nf][T][2]
@@ -438,8 +444,8 @@ struct BatchParams {
   const BatchChunk *chunks;
   const BatchItem *items;   // segments
   const int *work_start;    // item w = segments work_start[w] .. work_start[w + 1] - 1
-  const int *nitems;        // device: number of work items
-  unsigned *head;     // device: queue head, zeroed by the descriptor copy before each launch
+  const int *nitems;        // [number of work items, queue base, 1: queue / 0: one item per workgroup]
+  unsigned *head;           // device: the queue counter
   const v4i *W;       // channel table [C][KS][NTP][64]
   const int *corr;    // [C][BP][2]
   const float *G;     // [C][BP]
@@ -457,12 +463,22 @@ __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   v4i *Sl = mfma_stages_j(NT, KS) && a.stage_j ? Xl + kWaves * 64 : nullptr; // compiled out where it cannot fit
   int loaded_channel = -1, loaded_bg = -1;
-  const int nitems = *a.nitems;
-  for (;;) {
-    if (threadIdx.x == 0) s_item = (int)atomicAdd(a.head, 1u);
-    __syncthreads();
-    const int it = __builtin_amdgcn_readfirstlane(s_item);
-    if (it >= nitems) break; // uniform: every thread read the same s_item
+  const int nitems = a.nitems[0];
+  const unsigned base = (unsigned)a.nitems[1];
+  const bool queue = a.nitems[2] != 0; // uniform over the launch
+  for (int round = 0;; ++round) {
+    int it = blockIdx.x;
+    if (queue) {
+      if (threadIdx.x == 0) {
+        const unsigned u = atomicAdd(a.head, 1u) - base; // this launch's item index
+        s_item = u < (unsigned)nitems ? (int)u : nitems;
+      }
+      __syncthreads();
+      it = __builtin_amdgcn_readfirstlane(s_item);
+    } else if (round > 0) {
+      break;
+    }
+    if (it >= nitems) break; // uniform: the same value in every thread
     const int seg_end = a.work_start[it + 1];
     for (int sg = a.work_start[it]; sg < seg_end; ++sg) {
       const BatchItem item = a.items[sg];
@@ -488,7 +504,7 @@ __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a
       run_tiles<NT, KS, DT>(job, item.tile_begin, item.tile_end, Wl, Cl, Gl, Xl + wave * 64, Sl);
       __syncthreads(); // the LDS weights and staging are reused by the next segment
     }
-    __syncthreads(); // s_item is rewritten by the next pop
+    if (queue) __syncthreads(); // s_item is rewritten by the next pop
   }
 }
 

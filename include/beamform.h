@@ -164,8 +164,10 @@ typedef struct bf_chunk {
 typedef struct bf_batch bf_batch;
 
 /* Creates a batch context for up to max_chunks chunks of a channel-table
- * plan: device storage for the chunk descriptors and the work queue, and a
- * hipGraph of the launch, captured on first use of bf_batch_run_graph. The
+ * plan: two descriptor blocks in pinned host memory (chunk list and work
+ * items, written by alternate runs and read by the kernel in place), and a
+ * hipGraph of the launch per block, captured on first use of
+ * bf_batch_run_graph. The
  * batch follows its plan: after bf_plan_configure, bf_plan_autotune or
  * bf_plan_update_weights the next run re-derives its launch (and captures the
  * graph again); it fails with BF_ERR_UNSUPPORTED while new gains keep the
@@ -174,24 +176,27 @@ int bf_batch_create(const bf_plan *channel_plan, int max_chunks, bf_batch **batc
 int bf_batch_destroy(bf_batch *batch);
 
 /* Fixed work-item size in 16-sample tiles (rounded to a multiple of 8;
- * items taken longest first). 0 restores the default, balanced items: the
+ * items taken from the device-side queue longest first). 0 restores the
+ * default, balanced items (at most one per workgroup): the
  * chunks' tile ranges laid end to end and cut into one equal share per
  * workgroup of the persistent grid, counted in rounds of 8 tiles (a chunk's
  * last round may be partial), a share crossing chunk boundaries being run as
  * several segments. */
 int bf_batch_set_item_tiles(bf_batch *batch, int item_tiles);
 
-/* Runs all chunks in one persistent launch with a device-side work queue, on
- * `stream` (asynchronous; the chunk list is copied before the call returns).
- * Runs of one batch are ordered even across streams: each run's descriptor
- * copy waits for the previous run's launch. Use one batch per concurrent
- * launch. The plan must use the MFMA path (P = 2, D a multiple of 8 up to
- * 1024, gains in the fast range); otherwise BF_ERR_UNSUPPORTED. */
+/* Runs all chunks in one persistent launch on `stream` (balanced items one
+ * per workgroup, fixed-size items through a device-side work queue;
+ * asynchronous, the chunk list is read before the call returns).
+ * The launch needs no copy: the kernel reads the run's descriptor block from
+ * pinned host memory. A run
+ * first waits, on the host, until the run two before it on this batch has
+ * finished (it reuses that run's block); runs issued on different streams
+ * may overlap. The plan must use the MFMA path (P = 2, D a multiple of 8 up
+ * to 1024, gains in the fast range); otherwise BF_ERR_UNSUPPORTED. */
 int bf_batch_run(bf_batch *batch, const bf_chunk *chunks, int nchunks, void *stream);
 
-/* Same work, launched as a replay of the batch's captured hipGraph (the
- * persistent kernel); per call, one descriptor copy (chunk list, work items,
- * zeroed queue head) precedes the replay on `stream`. */
+/* Same work, launched as a replay of a captured hipGraph of the persistent
+ * kernel (one graph per descriptor block, captured on first use). */
 int bf_batch_run_graph(bf_batch *batch, const bf_chunk *chunks, int nchunks, void *stream);
 
 /* One-line description of the batch configuration. */
@@ -228,8 +233,8 @@ int bf_batch_autotune(bf_batch *batch, const bf_chunk *chunks, int nchunks, void
  * them in arrival order while a slot is free and the HBM budget (one device
  * arena allocated at creation) has room for the request's J and, for host-fed
  * E, its staging copy; it launches everything admitted as one persistent
- * batched launch (bf_batch_run; a lone request as bf_form_beams_channels,
- * which needs no descriptor copy) on one of max_launches streams, so requests
+ * batched launch (bf_batch_run; a lone request as bf_form_beams_channels)
+ * on one of max_launches streams, so requests
  * arriving while others run are taken into the next launch and overlap the
  * tail of the earlier one. J stays in the arena until bf_server_release.
  * The plan must outlive the server.

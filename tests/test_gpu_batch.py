@@ -118,7 +118,7 @@ def test_batch_graph_replays_match_eager():
 
 def test_batch_runs_on_two_streams_back_to_back():
     # Two runs of one batch on different streams with no synchronisation in
-    # between: the second run's descriptor copy waits for the first launch.
+    # between: they use the batch's two descriptor blocks and may overlap.
     C, D, B = 6, 512, 96
     A, G = channel_table(C, D, B, 13)
     shape = bf.Shape.make(1, C, D, B)
