@@ -122,8 +122,9 @@ WeightSet *set_of(bf_server *s, uint64_t gen) {
 
 // Issues one launch: staging copies of host-fed E, then the kernel, then the
 // completion event. Several requests go into one batched launch; a lone
-// request takes the plain channel-range launch, which needs no descriptor
-// copy before it (about 10 us less latency). Called without the server lock.
+// request takes the plain channel-range launch (no work items to build, and
+// the one-beam-tile configuration when it is small). Called without the
+// server lock.
 int issue(bf_server *s, Launch &L, const bf_plan *plan, bf_batch *batch, const std::vector<bf_chunk> &chunks,
           const std::vector<std::pair<uint8_t *, const Request *>> &copies) {
   for (const auto &c : copies) {
