@@ -1,0 +1,79 @@
+// bf_items.hpp -- the work items of a batched launch (host code, also
+// compiled for the CPU checks in tests/items_check.cpp): the chunks' time
+// tiles per (chunk, frequency, beam group) cut into segments, and segments
+// grouped into work items.
+#pragma once
+
+#include "beamform.h"
+#include "bf_device.hpp" // kTileT, kWaves
+
+#include <algorithm>
+#include <cstdint>
+#include <vector>
+
+namespace bf {
+
+// One segment: tiles tile_begin .. tile_end - 1 of frequency f of a chunk,
+// beam group bg.
+struct BatchItem {
+  int32_t chunk;
+  uint16_t f, bg; // frequency within the chunk, beam group
+  int32_t tile_begin, tile_end;
+};
+
+inline int chunk_nfreq(const bf_chunk &k) { return k.nfreq == 0 ? 1 : k.nfreq; }
+
+// Work items with a fixed size of `it` tiles, in chunk order.
+inline void fixed_items(const bf_chunk *chunks, int n, int nbg, int it, std::vector<BatchItem> &items) {
+  for (int i = 0; i < n; ++i) {
+    const int64_t nt = (chunks[i].ntimes + kTileT - 1) / kTileT;
+    const int nf = chunk_nfreq(chunks[i]);
+    for (int f = 0; f < nf; ++f)
+      for (int bg = 0; bg < nbg; ++bg)
+        for (int64_t t = 0; t < nt; t += it)
+          items.push_back(BatchItem{i, (uint16_t)f, (uint16_t)bg, (int32_t)t, (int32_t)std::min(nt, t + it)});
+  }
+}
+
+// Balanced work items. A workgroup runs a segment of m tiles in
+// ceil(m / 8) rounds (one tile per wave per round), so the cost of a
+// (chunk, frequency, beam group) piece is counted in rounds, its last one
+// possibly partial. The pieces are laid end to end and cut into one share of
+// ceil(total rounds / grid) rounds per workgroup; a cut inside a piece falls
+// on a round boundary, and a share that crosses a piece boundary becomes
+// several segments of one item. Every item then costs at most the share in
+// rounds, the E pipeline restarts only at segment starts, and the weights
+// are restaged only where a share crosses into another channel. Returns the
+// share in rounds.
+inline int64_t balanced_items(const bf_chunk *chunks, int n, int nbg, int grid, std::vector<BatchItem> &segs,
+                       std::vector<int> &work) {
+  int64_t total_rounds = 0;
+  for (int i = 0; i < n; ++i) {
+    const int64_t nt = (chunks[i].ntimes + kTileT - 1) / kTileT;
+    total_rounds += (int64_t)chunk_nfreq(chunks[i]) * nbg * ((nt + kWaves - 1) / kWaves);
+  }
+  const int64_t share = std::max<int64_t>(1, (total_rounds + grid - 1) / grid);
+  int64_t fill = 0;
+  work.push_back(0);
+  for (int i = 0; i < n; ++i) {
+    const int64_t nt = (chunks[i].ntimes + kTileT - 1) / kTileT;
+    const int nf = chunk_nfreq(chunks[i]);
+    for (int f = 0; f < nf; ++f)
+      for (int bg = 0; bg < nbg; ++bg)
+        for (int64_t t = 0; t < nt;) {
+          const int64_t rounds = std::min((nt - t + kWaves - 1) / kWaves, share - fill);
+          const int64_t take = std::min(nt - t, rounds * kWaves);
+          segs.push_back(BatchItem{i, (uint16_t)f, (uint16_t)bg, (int32_t)t, (int32_t)(t + take)});
+          t += take;
+          fill += rounds;
+          if (fill == share) {
+            work.push_back((int)segs.size());
+            fill = 0;
+          }
+        }
+  }
+  if (fill > 0) work.push_back((int)segs.size());
+  return share;
+}
+
+} // namespace bf

@@ -35,6 +35,7 @@
 #pragma once
 
 #include "bf_device.hpp"
+#include "bf_items.hpp"
 
 #include <type_traits>
 
@@ -434,11 +435,6 @@ struct BatchChunk {
   int64_t T;
   int32_t channel;  // channel-table row of frequency 0 of the chunk
   int32_t nf;       // frequencies in the chunk
-};
-struct BatchItem {
-  int32_t chunk;
-  uint16_t f, bg; // frequency within the chunk, beam group
-  int32_t tile_begin, tile_end;
 };
 struct BatchParams {
   const BatchChunk *chunks;
