@@ -383,7 +383,8 @@ def _chunk_array(chunks):
 
 class Batch:
     """A batch context over a channel-table plan: many chunks (each with its
-    own ntimes and channels) per persistent launch with a device-side queue."""
+    own ntimes and channels) per persistent launch (balanced work items, or
+    fixed-size items through a device-side queue)."""
 
     def __init__(self, plan, max_chunks):
         self.plan = plan  # keeps the plan alive
