@@ -250,3 +250,24 @@ def test_submitters_on_several_threads(table):
         for i, q in enumerate(reqs):
             want = oracle_request(q.E, A, G, q.T, q.channel, q.nf)
             assert np.array_equal(got[100 * k + i][1], want), f"thread {k} request {i}"
+
+
+def test_j2_layout_plan_through_the_server():
+    # a channel table with the [b][f][t][p] output layout: lone and batched
+    # launches both write J2, bit-exact with the oracle
+    A = bo.fill_uniform(C * B * D, 51)
+    G = bo.setup_gain(C * B, D) * np.float32(8)
+    reqs = random_reqs(12, 7)
+    torch.cuda.synchronize()
+    with bf.Plan(bf.Shape.make(1, C, D, B, j_layout=bf.BF_J_BFTP), A, G) as plan:
+        with bf.Server(plan, hbm_budget=1 << 30) as srv:
+            srv.submit(reqs[0].dE.data_ptr(), reqs[0].T, reqs[0].channel, reqs[0].nf, tag=0)  # alone
+            first = collect(srv, 1)
+            srv.submit_many([(q.dE.data_ptr(), q.T, q.channel, q.nf, i) for i, q in enumerate(reqs) if i > 0])
+            got = collect(srv, len(reqs) - 1)
+            got.update(first)
+    for i, q in enumerate(reqs):
+        Ac = A.reshape(C, B * D)[q.channel:q.channel + q.nf]
+        Gc = G.reshape(C, B)[q.channel:q.channel + q.nf]
+        want = bo.form_beams(q.E, Ac, Gc, q.T, q.nf, D, B, jlayout=1)
+        assert np.array_equal(got[i][1], want), f"request {i} (T={q.T}, channel={q.channel}, nf={q.nf})"
