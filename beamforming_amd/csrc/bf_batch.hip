@@ -30,6 +30,7 @@ struct bf_batch {
   bool last_queue = false; // the last prepared run takes its items from the queue
   int last_segments = 0;
   int grid = 0;
+  int grid_narrow = 0; // with the plan's one-beam-tile configuration (0: none)
   // Two descriptor blocks in pinned host memory, used by alternate runs and
   // read by the kernel in place: [work item count | pad to 16][chunks:
   // max_chunks][work item starts: item_capacity + 1, padded to 16][segments:
@@ -44,6 +45,8 @@ struct bf_batch {
     bool launch_recorded = false;
     hipGraph_t graph = nullptr; // captured launch reading this block
     hipGraphExec_t exec = nullptr;
+    bool narrow = false;        // the run written into the block takes one beam tile per workgroup
+    bool graph_narrow = false;  // the configuration the graph was captured with
   } blk[2];
   int next_block = 0;
   unsigned *d_heads = nullptr; // the two queue counters, 256 bytes apart
@@ -127,9 +130,27 @@ int follow_plan(bf_batch *b) {
       per_cu < 1)
     per_cu = 1;
   b->grid = p->num_cus * per_cu;
+  b->grid_narrow = 0;
+  if (p->has_narrow) {
+    const MfmaConfig &n = p->cfg_narrow;
+    BF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(n.batch_kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)mfma_lds_bytes(n.nt, n.ks)));
+    int per_cu_n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_n, reinterpret_cast<const void *>(n.batch_kernel),
+                                                     kThreads, n.lds_bytes) != hipSuccess ||
+        per_cu_n < 1)
+      per_cu_n = 1;
+    b->grid_narrow = p->num_cus * per_cu_n;
+  }
   b->plan_version = p->layout_version;
   return BF_OK;
 }
+
+// The configuration, grid and beam groups of a run in block j.
+const MfmaConfig &run_config(const bf_batch *b, int j) {
+  return b->blk[j].narrow ? b->plan->cfg_narrow : b->plan->cfg;
+}
+int run_grid(const bf_batch *b, int j) { return b->blk[j].narrow ? b->grid_narrow : b->grid; }
 
 // Validates the chunk list and writes it and the work items (longest first)
 // into the next descriptor block, whose index goes to *block.
@@ -154,6 +175,7 @@ int prepare(bf_batch *b, const bf_chunk *chunks, int n, int *block) {
   std::vector<BatchItem> items; // segments
   std::vector<int> work;        // work item starts
   int it = 0;
+  bool narrow = false;
   if (b->item_tiles > 0) {
     fixed_items(chunks, n, c.nbg, b->item_tiles, items);
     // longest first, one segment per work item
@@ -163,7 +185,16 @@ int prepare(bf_batch *b, const bf_chunk *chunks, int n, int *block) {
     for (int i = 0; i <= (int)items.size(); ++i) work.push_back(i);
     it = b->item_tiles;
   } else {
-    it = (int)(balanced_items(chunks, n, c.nbg, b->grid, items, work) * kWaves);
+    // Balanced items, one per workgroup. A batch too small to give every
+    // workgroup an item runs one beam tile per workgroup instead (the
+    // plan's one-beam-tile configuration over the same weight tables), with
+    // NT times the items.
+    int64_t rounds = 0;
+    for (int i = 0; i < n; ++i)
+      rounds += (int64_t)chunk_nfreq(chunks[i]) * c.nbg * (((chunks[i].ntimes + kTileT - 1) / kTileT + kWaves - 1) / kWaves);
+    narrow = b->grid_narrow > 0 && rounds < b->grid;
+    const MfmaConfig &cu = narrow ? p->cfg_narrow : c;
+    it = (int)(balanced_items(chunks, n, cu.nbg, narrow ? b->grid_narrow : b->grid, items, work) * kWaves);
   }
   const int nwork = (int)work.size() - 1;
   if (int rc = ensure_item_capacity(b, std::max<int>(1, (int)items.size()))) return rc;
@@ -179,6 +210,7 @@ int prepare(bf_batch *b, const bf_chunk *chunks, int n, int *block) {
   reinterpret_cast<uint32_t *>(k.h)[1] = k.base;
   reinterpret_cast<int *>(k.h)[2] = b->item_tiles > 0 ? 1 : 0; // balanced items: one per workgroup, no queue
   b->last_queue = b->item_tiles > 0;
+  k.narrow = narrow;
   std::atomic_thread_fence(std::memory_order_seq_cst); // the block is written before the launch is enqueued
   b->next_block = j ^ 1;
   *block = j;
@@ -200,20 +232,20 @@ BatchParams batch_params(const bf_batch *b, int j) {
   a.W = p->W;
   a.corr = p->corr;
   a.G = p->Gp;
-  a.NTP = p->cfg.ntp;
+  a.NTP = p->cfg.ntp; // the one-beam-tile configuration shares the tables
   a.D = (int32_t)p->shape.ndishes;
   a.B = (int32_t)p->shape.nbeams;
   a.jlayout = p->shape.j_layout;
-  a.stage_j = p->cfg.stage_j ? 1 : 0;
+  a.stage_j = run_config(b, j).stage_j ? 1 : 0;
   return a;
 }
 
 // The launch itself: the persistent kernel reading descriptor block j.
 int enqueue_launch(bf_batch *b, int j, hipStream_t st) {
-  const MfmaConfig &c = b->plan->cfg;
+  const MfmaConfig &c = run_config(b, j);
   BatchParams a = batch_params(b, j);
   void *args[] = {&a};
-  BF_HIP(hipLaunchKernel(reinterpret_cast<const void *>(c.batch_kernel), dim3(b->grid), dim3(kThreads), args,
+  BF_HIP(hipLaunchKernel(reinterpret_cast<const void *>(c.batch_kernel), dim3(run_grid(b, j)), dim3(kThreads), args,
                          c.lds_bytes, st));
   return BF_OK;
 }
@@ -221,7 +253,7 @@ int enqueue_launch(bf_batch *b, int j, hipStream_t st) {
 // After a launch on block j was enqueued: its queue counter will have grown
 // by the item count plus one failed fetch per workgroup.
 int record_launch(bf_batch *b, int j, hipStream_t st) {
-  if (b->last_queue) b->blk[j].base += (uint32_t)(b->last_nitems + b->grid);
+  if (b->last_queue) b->blk[j].base += (uint32_t)(b->last_nitems + run_grid(b, j));
   BF_HIP(hipEventRecord(b->blk[j].launch_done, st));
   b->blk[j].launch_recorded = true;
   return BF_OK;
@@ -338,6 +370,12 @@ int bf_batch_run_graph(bf_batch *b, const bf_chunk *chunks, int nchunks, void *s
   int j = 0;
   if (int rc = prepare(b, chunks, nchunks, &j)) return rc;
   bf_batch::Block &k = b->blk[j];
+  if (k.exec && k.graph_narrow != k.narrow) { // captured with the other configuration
+    (void)hipGraphExecDestroy(k.exec);
+    (void)hipGraphDestroy(k.graph);
+    k.exec = nullptr;
+    k.graph = nullptr;
+  }
   if (!k.exec) {
     // Capture once per block: the launch reads chunks, items and their count
     // from the block, so the graph stays valid for any chunk list.
@@ -351,6 +389,7 @@ int bf_batch_run_graph(bf_batch *b, const bf_chunk *chunks, int nchunks, void *s
     }
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
     k.graph = g;
+    k.graph_narrow = k.narrow;
     BF_HIP(hipGraphInstantiate(&k.exec, k.graph, nullptr, nullptr, 0));
   }
   BF_HIP(hipGraphLaunch(k.exec, st));
@@ -359,13 +398,15 @@ int bf_batch_run_graph(bf_batch *b, const bf_chunk *chunks, int nchunks, void *s
 
 int bf_batch_describe(const bf_batch *b, char *buf, size_t len) {
   if (!b || !buf || len == 0) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
-  const MfmaConfig &c = b->plan->cfg;
+  const int last = b->next_block ^ 1; // the block of the last run
+  const MfmaConfig &c = run_config(b, last);
   std::snprintf(buf, len,
                 "persistent mfma_i32_32x32x32_i8 batch: nt=%d ks=%d nbg=%d stage_j=%d grid=%d lds=%zu item_tiles=%d%s "
                 "items=%d "
                 "segments=%d "
                 "graph=%s",
-                c.nt, c.ks, c.nbg, c.stage_j ? 1 : 0, b->grid, c.lds_bytes, b->last_item_tiles, b->item_tiles ? "" : " (balanced)",
+                c.nt, c.ks, c.nbg, c.stage_j ? 1 : 0, run_grid(b, last), c.lds_bytes, b->last_item_tiles,
+                b->item_tiles ? "" : " (balanced)",
                 b->last_nitems, b->last_segments, b->blk[0].exec || b->blk[1].exec ? "captured" : "none");
   return BF_OK;
 }

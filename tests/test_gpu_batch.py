@@ -330,3 +330,28 @@ def test_small_launches_with_one_beam_tile_per_workgroup(D, B, layout):
         plan2.form_beams_device(dJ2.data_ptr(), dE2.data_ptr(), stream())
         torch.cuda.synchronize()
     assert np.array_equal(dJ2.cpu().numpy(), bo.form_beams(E2, A[:F * B * D], G[:F * B], T, F, D, B, jlayout=layout))
+
+
+def test_small_batches_take_one_beam_tile_per_workgroup():
+    # a balanced batch too small to give every workgroup an item runs one beam
+    # tile per workgroup; alternating small and large batches (eager and
+    # graph replay, whose graphs are captured again when the choice changes)
+    # stay exact
+    C, D, B = 6, 512, 96
+    A, G = channel_table(C, D, B, 43)
+    small = [(2048, 0), (700, 3), (16, 5)]
+    large = [(20000, c % C) for c in range(12)]
+    hs, ds = make_chunks(small, D, B, 1100)
+    hl, dl = make_chunks(large, D, B, 1200)
+    with bf.Plan(bf.Shape.make(1, C, D, B), A, G) as plan, bf.Batch(plan, 16) as batch:
+        for graph in (False, True, True, False, True):
+            for specs, host, dev, want_nt in ((small, hs, ds, 1), (large, hl, dl, 3)):
+                for _, dJ in dev:
+                    dJ.zero_()
+                torch.cuda.synchronize()
+                batch.run(chunk_list(specs, dev), stream(), graph=graph)
+                torch.cuda.synchronize()
+                desc = batch.describe()
+                say(f"  graph={graph}: {desc}")
+                assert f"nt={want_nt} " in desc
+                check_chunks(specs, host, dev, A, G, D, B)
