@@ -191,8 +191,10 @@ int bf_batch_set_item_tiles(bf_batch *batch, int item_tiles);
  * pinned host memory. A run
  * first waits, on the host, until the run two before it on this batch has
  * finished (it reuses that run's block); runs issued on different streams
- * may overlap. The plan must use the MFMA path (P = 2, D a multiple of 8 up
- * to 1024, gains in the fast range); otherwise BF_ERR_UNSUPPORTED. */
+ * may overlap. A balanced batch too small to give every workgroup an item
+ * runs one beam tile per workgroup (plans with the automatic configuration).
+ * The plan must use the MFMA path (P = 2, D a multiple of 8 up to 1024, gains
+ * in the fast range); otherwise BF_ERR_UNSUPPORTED. */
 int bf_batch_run(bf_batch *batch, const bf_chunk *chunks, int nchunks, void *stream);
 
 /* Same work, launched as a replay of a captured hipGraph of the persistent
