@@ -1,7 +1,9 @@
 // bf_server.hip -- the streaming request loop (continuous batching).
 //
 // Requests (one chunk each: E of nfreq consecutive channels, own ntimes) are
-// submitted from any thread. A loop thread owned by the server
+// submitted from any thread (with poll_us 0 the submitting thread itself
+// admits and launches when a launch slot is free). A loop thread owned by the
+// server
 //   1. admits waiting requests in arrival order while a slot is free and the
 //      HBM arena (the budget, one device allocation made at creation) has
 //      room for the request's J and, for host-fed E, its staging copy;
@@ -184,6 +186,56 @@ void admit_locked(bf_server *s) {
   });
 }
 
+// Launches admitted requests (one weight set, up to max_batch) on a free
+// launch slot; returns false when there is no slot or nothing to launch.
+// Called with the lock held; drops it around the launch itself.
+bool launch_locked(bf_server *s, std::unique_lock<std::mutex> &lk) {
+  int slot = -1;
+  for (size_t i = 0; i < s->launches.size(); ++i)
+    if (!s->launches[i].busy) {
+      slot = (int)i;
+      break;
+    }
+  if (slot < 0 || s->ready.empty() || s->stopping) return false;
+  Launch &L = s->launches[slot];
+  std::vector<bf_chunk> chunks;
+  std::vector<std::pair<uint8_t *, const Request *>> copies;
+  const double t = s->now();
+  // one weight set per launch: the oldest waiting request's
+  const uint64_t gen = s->reqs[s->ready.front()].gen;
+  WeightSet *wset = set_of(s, gen); // a set with waiting requests is never freed
+  bf_batch *batch = wset->batches[slot];
+  const bf_plan *plan = wset->plan;
+  while (!s->ready.empty() && (int)chunks.size() < s->cfg.max_batch && s->reqs[s->ready.front()].gen == gen) {
+    const uint64_t id = s->ready.front();
+    s->ready.pop_front();
+    Request &r = s->reqs[id];
+    r.t_launch = t;
+    const uint8_t *E = r.E_src;
+    if (r.e_on_host) {
+      E = s->arena + r.e_off;
+      copies.emplace_back(s->arena + r.e_off, &r);
+    }
+    chunks.push_back(bf_chunk{E, s->arena + r.j_off, r.ntimes, r.channel, r.nfreq});
+    L.tickets.push_back(id);
+  }
+  for (uint64_t id : L.tickets) s->reqs[id].batch_size = (int)L.tickets.size();
+  L.busy = true; // the slot is ours while the lock is dropped
+  ++s->n_launches;
+  s->batch_requests += L.tickets.size();
+  s->max_batch_seen = std::max<uint64_t>(s->max_batch_seen, L.tickets.size());
+  lk.unlock();
+  const int rc = issue(s, L, plan, batch, chunks, copies); // Request pointers stay valid: entries are only erased on release
+  if (rc) (void)hipStreamSynchronize(L.stream); // no staging copy of a failed launch may still write the arena
+  lk.lock();
+  if (rc) {
+    s->loop_error = g_last_error;
+    finish(s, L, rc, s->now());
+    s->cv_done.notify_all();
+  }
+  return true;
+}
+
 void run_loop(bf_server *s) {
   if (hipSetDevice(s->device) != hipSuccess) {
     std::lock_guard<std::mutex> lk(s->mu);
@@ -194,52 +246,7 @@ void run_loop(bf_server *s) {
     bool progressed = false;
     if (!s->stopping) admit_locked(s); // shutting down: only finish what is on the device
 
-    // Launch what is admitted, on a free launch slot.
-    int slot = -1;
-    for (size_t i = 0; i < s->launches.size(); ++i)
-      if (!s->launches[i].busy) {
-        slot = (int)i;
-        break;
-      }
-    if (slot >= 0 && !s->ready.empty() && !s->stopping) {
-      Launch &L = s->launches[slot];
-      std::vector<bf_chunk> chunks;
-      std::vector<std::pair<uint8_t *, const Request *>> copies;
-      const double t = s->now();
-      // one weight set per launch: the oldest waiting request's
-      const uint64_t gen = s->reqs[s->ready.front()].gen;
-      WeightSet *wset = set_of(s, gen); // a set with waiting requests is never freed
-      bf_batch *batch = wset->batches[slot];
-      const bf_plan *plan = wset->plan;
-      while (!s->ready.empty() && (int)chunks.size() < s->cfg.max_batch && s->reqs[s->ready.front()].gen == gen) {
-        const uint64_t id = s->ready.front();
-        s->ready.pop_front();
-        Request &r = s->reqs[id];
-        r.t_launch = t;
-        const uint8_t *E = r.E_src;
-        if (r.e_on_host) {
-          E = s->arena + r.e_off;
-          copies.emplace_back(s->arena + r.e_off, &r);
-        }
-        chunks.push_back(bf_chunk{E, s->arena + r.j_off, r.ntimes, r.channel, r.nfreq});
-        L.tickets.push_back(id);
-      }
-      for (uint64_t id : L.tickets) s->reqs[id].batch_size = (int)L.tickets.size();
-      L.busy = true;
-      ++s->n_launches;
-      s->batch_requests += L.tickets.size();
-      s->max_batch_seen = std::max<uint64_t>(s->max_batch_seen, L.tickets.size());
-      lk.unlock();
-      const int rc = issue(s, L, plan, batch, chunks, copies); // Request pointers stay valid: entries are only erased on release
-      if (rc) (void)hipStreamSynchronize(L.stream); // no staging copy of a failed launch may still write the arena
-      lk.lock();
-      if (rc) {
-        s->loop_error = g_last_error;
-        finish(s, L, rc, s->now());
-        s->cv_done.notify_all();
-      }
-      progressed = true;
-    }
+    if (launch_locked(s, lk)) progressed = true;
 
     // Completion.
     bool any_busy = false;
@@ -398,7 +405,7 @@ int bf_server_submit_many(bf_server *s, const bf_request *reqs, int n, uint64_t 
     totals[i] = s->alloc.round(needs[i].bytes[0]) + (needs[i].bytes[1] ? s->alloc.round(needs[i].bytes[1]) : 0);
   }
   {
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::unique_lock<std::mutex> lk(s->mu);
     if (s->stopping) return fail(BF_ERR_CLOSED, "server is shutting down");
     for (int i = 0; i < n; ++i)
       if (totals[i] > s->alloc.capacity()) {
@@ -418,6 +425,17 @@ int bf_server_submit_many(bf_server *s, const bf_request *reqs, int n, uint64_t 
       s->adm->push(r.ticket, needs[i]);
       ++s->n_submitted;
       if (tickets) tickets[i] = r.ticket;
+    }
+    if (s->cfg.poll_us == 0) {
+      // Latency first: admit and launch from the submitting thread when a
+      // launch slot is free, instead of waiting for the loop thread to wake
+      // (about 5-10 us); the loop then polls the launch's completion.
+      admit_locked(s);
+      int prev = -1;
+      if (hipGetDevice(&prev) == hipSuccess && (prev == s->device || hipSetDevice(s->device) == hipSuccess)) {
+        launch_locked(s, lk);
+        if (prev != s->device) (void)hipSetDevice(prev); // leave the caller's current device as it was
+      }
     }
   }
   s->cv_work.notify_all();

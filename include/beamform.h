@@ -250,10 +250,12 @@ typedef struct bf_server_config {
   int32_t max_batch;    /* requests per launch (0 = 128) */
   int32_t max_launches; /* launches in flight at once, one stream each (0 = 2) */
   int32_t use_graph;    /* 1: each launch of several requests is a replay of a captured hipGraph */
-  int32_t poll_us;      /* while launches are in flight, the loop thread polls their completion:
-                           0 (default) without sleeping, yielding its core (lowest latency; one core
-                           stays busy while the GPU works); N in 1..1000000 sleeps up to N us between
-                           polls (the OS may wake it tens of microseconds late) */
+  int32_t poll_us;      /* latency against CPU use. 0 (default): while launches are in flight the
+                           loop thread polls their completion without sleeping, yielding its core
+                           (one core stays busy while the GPU works), and a submission is launched
+                           at once from the submitting thread when a launch slot is free.
+                           N in 1..1000000: the loop sleeps up to N us between polls (the OS may wake
+                           it tens of microseconds late) and launches everything itself */
 } bf_server_config;
 
 typedef struct bf_request {
