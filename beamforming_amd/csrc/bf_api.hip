@@ -64,26 +64,26 @@ static bool gains_fast_safe(const float *G, int64_t n, int64_t D) {
   return true;
 }
 
-static MfmaKernel lookup_mfma(int nt, int ks, bool dt) {
+static MfmaKernel lookup_mfma(int nt, int ks, bool dt, int jl) {
   switch (ks) {
-  case 2: return mfma_kernel_ks2(nt, dt);
-  case 4: return mfma_kernel_ks4(nt, dt);
-  case 8: return mfma_kernel_ks8(nt, dt);
-  case 16: return mfma_kernel_ks16(nt, dt);
-  case 32: return mfma_kernel_ks32(nt, dt);
-  case 64: return mfma_kernel_ks64(nt, dt);
+  case 2: return mfma_kernel_ks2(nt, dt, jl);
+  case 4: return mfma_kernel_ks4(nt, dt, jl);
+  case 8: return mfma_kernel_ks8(nt, dt, jl);
+  case 16: return mfma_kernel_ks16(nt, dt, jl);
+  case 32: return mfma_kernel_ks32(nt, dt, jl);
+  case 64: return mfma_kernel_ks64(nt, dt, jl);
   default: return nullptr;
   }
 }
 
-BatchKernel lookup_batch(int nt, int ks, bool dt) {
+BatchKernel lookup_batch(int nt, int ks, bool dt, int jl) {
   switch (ks) {
-  case 2: return batch_kernel_ks2(nt, dt);
-  case 4: return batch_kernel_ks4(nt, dt);
-  case 8: return batch_kernel_ks8(nt, dt);
-  case 16: return batch_kernel_ks16(nt, dt);
-  case 32: return batch_kernel_ks32(nt, dt);
-  case 64: return batch_kernel_ks64(nt, dt);
+  case 2: return batch_kernel_ks2(nt, dt, jl);
+  case 4: return batch_kernel_ks4(nt, dt, jl);
+  case 8: return batch_kernel_ks8(nt, dt, jl);
+  case 16: return batch_kernel_ks16(nt, dt, jl);
+  case 32: return batch_kernel_ks32(nt, dt, jl);
+  case 64: return batch_kernel_ks64(nt, dt, jl);
   default: return nullptr;
   }
 }
@@ -138,19 +138,21 @@ bool choose_tiling(int64_t ndishes, int64_t nbeams, int forced_nt, MfmaConfig *c
   cfg->nt = best_nt;
   cfg->nbg = best_nbg;
   cfg->ntp = best_nt * best_nbg;
-  choose_lds_layout(0, true, cfg);
   cfg->dish_tail = (int64_t)ks * kDishesPerStep != ndishes;
-  cfg->kernel = lookup_mfma(best_nt, ks, cfg->dish_tail);
-  cfg->batch_kernel = lookup_batch(best_nt, ks, cfg->dish_tail);
+  choose_lds_layout(BF_J_BFPT, true, cfg); // callers set the plan's layout next
   return cfg->kernel != nullptr && cfg->batch_kernel != nullptr;
 }
 
-// LDS layout: the J staging area is used with the [b][f][p][t] layout when it
-// fits beside the weights (and when wanted); without it the workgroup needs
-// less LDS, which can let more workgroups share a CU.
+// LDS layout: the J staging area is used (with either J layout) when it fits
+// beside the weights and when wanted; without it the workgroup needs less
+// LDS, which can let more workgroups share a CU.
 void choose_lds_layout(int j_layout, bool want_staging, MfmaConfig *cfg) {
-  cfg->stage_j = want_staging && j_layout == BF_J_BFPT && mfma_stages_j(cfg->nt, cfg->ks);
+  // both layouts stage whole rows: (beam, polarization) x 128 B or beam x 256 B
+  cfg->stage_j = want_staging && mfma_stages_j(cfg->nt, cfg->ks);
   cfg->lds_bytes = mfma_lds_unstaged(cfg->nt, cfg->ks) + (cfg->stage_j ? mfma_lds_staging(cfg->nt) : 0);
+  // the kernels of the configuration, with the J layout's epilogue
+  cfg->kernel = lookup_mfma(cfg->nt, cfg->ks, cfg->dish_tail, j_layout);
+  cfg->batch_kernel = lookup_batch(cfg->nt, cfg->ks, cfg->dish_tail, j_layout);
 }
 
 // Grid for T time samples and F frequencies: about target_wgs workgroups, each

@@ -17,17 +17,17 @@ __global__ void zero_u64_kernel(unsigned long long *p, int n);
 __global__ void fill_uniform_kernel(uint8_t *__restrict__ buf, uint64_t n, uint64_t seed, uint64_t offset);
 
 // MFMA kernel instances, one translation unit per K-step count.
-MfmaKernel mfma_kernel_ks2(int nt, bool dish_tail);
-MfmaKernel mfma_kernel_ks4(int nt, bool dish_tail);
-MfmaKernel mfma_kernel_ks8(int nt, bool dish_tail);
-MfmaKernel mfma_kernel_ks16(int nt, bool dish_tail);
-MfmaKernel mfma_kernel_ks32(int nt, bool dish_tail);
-MfmaKernel mfma_kernel_ks64(int nt, bool dish_tail);
-BatchKernel batch_kernel_ks2(int nt, bool dish_tail);
-BatchKernel batch_kernel_ks4(int nt, bool dish_tail);
-BatchKernel batch_kernel_ks8(int nt, bool dish_tail);
-BatchKernel batch_kernel_ks16(int nt, bool dish_tail);
-BatchKernel batch_kernel_ks32(int nt, bool dish_tail);
-BatchKernel batch_kernel_ks64(int nt, bool dish_tail);
+MfmaKernel mfma_kernel_ks2(int nt, bool dish_tail, int jl);
+MfmaKernel mfma_kernel_ks4(int nt, bool dish_tail, int jl);
+MfmaKernel mfma_kernel_ks8(int nt, bool dish_tail, int jl);
+MfmaKernel mfma_kernel_ks16(int nt, bool dish_tail, int jl);
+MfmaKernel mfma_kernel_ks32(int nt, bool dish_tail, int jl);
+MfmaKernel mfma_kernel_ks64(int nt, bool dish_tail, int jl);
+BatchKernel batch_kernel_ks2(int nt, bool dish_tail, int jl);
+BatchKernel batch_kernel_ks4(int nt, bool dish_tail, int jl);
+BatchKernel batch_kernel_ks8(int nt, bool dish_tail, int jl);
+BatchKernel batch_kernel_ks16(int nt, bool dish_tail, int jl);
+BatchKernel batch_kernel_ks32(int nt, bool dish_tail, int jl);
+BatchKernel batch_kernel_ks64(int nt, bool dish_tail, int jl);
 
 } // namespace bf

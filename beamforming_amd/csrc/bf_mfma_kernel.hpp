@@ -23,11 +23,13 @@
 //    l and l + 32 (v_permlane32_swap), so that each lane holds the 16 bytes
 //    of 16 consecutive times of one (beam, polarization) row;
 //  * the 8 waves advance in rounds of 8 consecutive tiles. When the launch
-//    stages J (the [b][f][p][t] layout, room in LDS, and the configuration
-//    asks for it), each wave writes its rows into a double-buffered LDS
-//    staging area and, after the round's barrier, the workgroup stores every
-//    row's 128 bytes (8 tiles) with whole-line writes; otherwise each lane
-//    stores its 16 bytes directly.
+//    stages J (room in LDS, and the configuration asks for it), each wave
+//    writes its rows into a double-buffered LDS staging area and, after the
+//    round's barrier, the workgroup stores every row's bytes of the round
+//    (8 tiles: 128 bytes per (beam, polarization) row of [b][f][p][t], 256
+//    bytes per beam row of [b][f][t][p]) with whole-line writes; otherwise
+//    each lane stores its pieces directly. The J layout is a template
+//    parameter (JL), so each kernel holds one epilogue.
 // Rows past T and dishes past D are clamped, always-in-bounds loads: the
 // outputs of rows past T are never stored and the weights of dishes past D
 // are zero. The plan routes gains outside the fast requantisation range
@@ -88,9 +90,9 @@ __device__ __forceinline__ void store_row8(uint8_t *dst, uint32_t w0, uint32_t w
 
 // LDS of a workgroup: [KS][NT][64] weight operands, the beam-column nibble
 // offsets and gains of the beam group, one 1 KiB E transpose buffer per wave,
-// then, when the launch stages J (the [b][f][p][t] layout, and it fits in the
-// 160 KiB of a CU), the J staging area: 2 rounds x (NT x 32 beams x 2
-// polarizations) rows x 128 bytes.
+// then, when the launch stages J (and it fits in the 160 KiB of a CU), the J
+// staging area: 2 rounds x (NT x 32 beams x 2 polarizations) rows x 128 bytes
+// ([b][f][p][t]), or 2 rounds x NT x 32 beam rows x 256 bytes ([b][f][t][p]).
 constexpr size_t kLdsPerCu = 160 * 1024;
 BF_HD constexpr size_t mfma_lds_unstaged(int nt, int ks) {
   return (size_t)ks * nt * 64 * sizeof(v4i) + (size_t)nt * 32 * (sizeof(int2) + sizeof(float)) +
@@ -149,7 +151,7 @@ __device__ __forceinline__ void store_piece(uint8_t *dst, const v4i v, int n, bo
 // wave in round r), against the weights staged in LDS (Wl, Cl, Gl), using
 // the wave's transpose buffer xw. Sl is the J staging area (NULL: direct
 // stores). All waves of the workgroup call this with the same arguments.
-template <int NT, int KS, bool DT>
+template <int NT, int KS, bool DT, int JL>
 __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile_first, const int64_t tile_end,
                                           const v4i *Wl, const int2 *Cl, const float *Gl, v4i *xw, v4i *Sl) {
   static_assert(NT >= 1 && NT <= 4, "32-beam tiles per workgroup");
@@ -301,7 +303,7 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
         wv[nt][k] = pack_low_bytes(b[0], b[1], b[2], b[3]);
       }
     }
-    if (job.jlayout == 0) {
+    if constexpr (JL == 0) {
       // J[b][f][p][t]: v_permlane32_swap exchanges lanes 32..63 of its first
       // operand with lanes 0..31 of its second. Swapping (p0 t0-3 | p1 t0-3)
       // pairs and (p0 t8-11 | p1 t8-11) pairs leaves lane h = 0 with the
@@ -336,6 +338,26 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
             store_piece(job.J + (((int64_t)b * job.F + job.f) * 2 + h) * job.T + t0, row[nt], nvalid, fast_store);
         }
       }
+    } else if (Sl) {
+      // J[b][f][t][p], staged: interleave the polarizations of the lane's
+      // two groups of 4 times (A: times 4h.., B: times 8 + 4h..), then swap
+      // halves between lanes l and l + 32 so that lane h holds the 16 bytes
+      // of times 8h .. 8h + 7 of its beam (h = 0: its A and lane l + 32's A;
+      // h = 1: lane l - 32's B and its B). Row = beam, 256 bytes per round
+      // (8 tiles x 32 bytes); the 16-byte piece slots are rotated by the row
+      // so that 16 consecutive lanes hit distinct banks.
+      v4i *buf = Sl + (size_t)(r & 1) * (NT * 64 * 8);
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const uint32_t alo = perm_bytes(wv[nt][2], wv[nt][0], 0x05010400u);
+        const uint32_t ahi = perm_bytes(wv[nt][2], wv[nt][0], 0x07030602u);
+        const uint32_t blo = perm_bytes(wv[nt][3], wv[nt][1], 0x05010400u);
+        const uint32_t bhi = perm_bytes(wv[nt][3], wv[nt][1], 0x07030602u);
+        const auto lo = __builtin_amdgcn_permlane32_swap(alo, blo, false, false);
+        const auto hi = __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false);
+        const int srow = nt * 32 + col;
+        buf[srow * 16 + ((2 * wave + h) ^ (srow & 15))] = v4i{(int)lo[0], (int)hi[0], (int)lo[1], (int)hi[1]};
+      }
     } else {
       __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0), as above
 #pragma unroll
@@ -368,22 +390,38 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
     const int64_t t_round = (tile_first + r * step) * kTileT;
     const int64_t t_end = tile_end * kTileT < job.T ? tile_end * kTileT : job.T;
     __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0), as above
+    if constexpr (JL == 0) {
 #pragma unroll
-    for (int k = 0; k < NT; ++k) {
-      const int q = threadIdx.x + kThreads * k;
-      const int srow = q >> 3, seg = q & 7;
-      const v4i v = buf[srow * 8 + (seg ^ ((srow >> 1) & 7))];
-      const int b = job.bg * NT * 32 + (srow >> 1);
-      const int64_t t = t_round + 16 * seg;
-      if (b < job.B && t < t_end)
-        store_piece(job.J + (((int64_t)b * job.F + job.f) * 2 + (srow & 1)) * job.T + t, v,
-                    t_end - t < 16 ? (int)(t_end - t) : 16, fast_store);
+      for (int k = 0; k < NT; ++k) {
+        const int q = threadIdx.x + kThreads * k;
+        const int srow = q >> 3, seg = q & 7;
+        const v4i v = buf[srow * 8 + (seg ^ ((srow >> 1) & 7))];
+        const int b = job.bg * NT * 32 + (srow >> 1);
+        const int64_t t = t_round + 16 * seg;
+        if (b < job.B && t < t_end)
+          store_piece(job.J + (((int64_t)b * job.F + job.f) * 2 + (srow & 1)) * job.T + t, v,
+                      t_end - t < 16 ? (int)(t_end - t) : 16, fast_store);
+      }
+    } else {
+      // [b][f][t][p]: thread q writes piece q & 15 (8 times, 16 bytes) of
+      // beam row q >> 4; 16 consecutive threads store one row's 256 bytes.
+#pragma unroll
+      for (int k = 0; k < NT; ++k) {
+        const int q = threadIdx.x + kThreads * k;
+        const int srow = q >> 4, seg = q & 15;
+        const v4i v = buf[srow * 16 + (seg ^ (srow & 15))];
+        const int b = job.bg * NT * 32 + srow;
+        const int64_t t = t_round + 8 * seg;
+        if (b < job.B && t < t_end)
+          store_piece(job.J + (((int64_t)b * job.F + job.f) * job.T + t) * 2, v,
+                      t_end - t < 8 ? (int)(2 * (t_end - t)) : 16, fast_store);
+      }
     }
   }
   } // rounds
 }
 
-template <int NT, int KS, bool DT>
+template <int NT, int KS, bool DT, int JL>
 __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) {
   extern __shared__ v4i Wl[];
   int2 *Cl = reinterpret_cast<int2 *>(Wl + KS * NT * 64);
@@ -412,7 +450,7 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
   job.bg = wi.bg;
   job.jlayout = a.jlayout;
   v4i *Sl = mfma_stages_j(NT, KS) && a.stage_j ? Xl + kWaves * 64 : nullptr; // compiled out where it cannot fit
-  run_tiles<NT, KS, DT>(job, tile_begin, tile_end, Wl, Cl, Gl, Xl + wave * 64, Sl);
+  run_tiles<NT, KS, DT, JL>(job, tile_begin, tile_end, Wl, Cl, Gl, Xl + wave * 64, Sl);
 }
 
 // ---------------------------------------------------------------------------
@@ -449,7 +487,7 @@ struct BatchParams {
   int32_t stage_j; // 1: J rows go through the LDS staging area (allocated in the launch's LDS)
 };
 
-template <int NT, int KS, bool DT>
+template <int NT, int KS, bool DT, int JL>
 __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a) {
   extern __shared__ v4i Wl[];
   int2 *Cl = reinterpret_cast<int2 *>(Wl + KS * NT * 64);
@@ -497,7 +535,7 @@ __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a
       job.B = a.B;
       job.bg = item.bg;
       job.jlayout = a.jlayout;
-      run_tiles<NT, KS, DT>(job, item.tile_begin, item.tile_end, Wl, Cl, Gl, Xl + wave * 64, Sl);
+      run_tiles<NT, KS, DT, JL>(job, item.tile_begin, item.tile_end, Wl, Cl, Gl, Xl + wave * 64, Sl);
       __syncthreads(); // the LDS weights and staging are reused by the next segment
     }
     if (queue) __syncthreads(); // s_item is rewritten by the next pop
@@ -509,5 +547,17 @@ __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a
 // KS * 16 (the last K steps are partly or wholly past D).
 typedef void (*MfmaKernel)(const MfmaParams);
 typedef void (*BatchKernel)(const BatchParams);
+
+// The instances of one (NT, KS): dish tail or not, J layout 0 ([b][f][p][t])
+// or 1 ([b][f][t][p]). The layout is a template parameter so that each
+// kernel holds one epilogue and its register budget serves that one alone.
+template <int NT, int KS> MfmaKernel mfma_instance(bool dt, int jl) {
+  return dt ? (jl ? form_beams_mfma<NT, KS, true, 1> : form_beams_mfma<NT, KS, true, 0>)
+            : (jl ? form_beams_mfma<NT, KS, false, 1> : form_beams_mfma<NT, KS, false, 0>);
+}
+template <int NT, int KS> BatchKernel batch_instance(bool dt, int jl) {
+  return dt ? (jl ? form_beams_batch<NT, KS, true, 1> : form_beams_batch<NT, KS, true, 0>)
+            : (jl ? form_beams_batch<NT, KS, false, 1> : form_beams_batch<NT, KS, false, 0>);
+}
 
 } // namespace bf

@@ -4,31 +4,31 @@
 
 namespace bf {
 
-MfmaKernel mfma_kernel_ks16(int nt, bool dish_tail) {
+MfmaKernel mfma_kernel_ks16(int nt, bool dish_tail, int jl) {
   switch (nt) {
   case 1:
-    return dish_tail ? form_beams_mfma<1, 16, true> : form_beams_mfma<1, 16, false>;
+    return mfma_instance<1, 16>(dish_tail, jl);
   case 2:
-    return dish_tail ? form_beams_mfma<2, 16, true> : form_beams_mfma<2, 16, false>;
+    return mfma_instance<2, 16>(dish_tail, jl);
   case 3:
-    return dish_tail ? form_beams_mfma<3, 16, true> : form_beams_mfma<3, 16, false>;
+    return mfma_instance<3, 16>(dish_tail, jl);
   case 4:
-    return dish_tail ? form_beams_mfma<4, 16, true> : form_beams_mfma<4, 16, false>;
+    return mfma_instance<4, 16>(dish_tail, jl);
   default:
     return nullptr;
   }
 }
 
-BatchKernel batch_kernel_ks16(int nt, bool dish_tail) {
+BatchKernel batch_kernel_ks16(int nt, bool dish_tail, int jl) {
   switch (nt) {
   case 1:
-    return dish_tail ? form_beams_batch<1, 16, true> : form_beams_batch<1, 16, false>;
+    return batch_instance<1, 16>(dish_tail, jl);
   case 2:
-    return dish_tail ? form_beams_batch<2, 16, true> : form_beams_batch<2, 16, false>;
+    return batch_instance<2, 16>(dish_tail, jl);
   case 3:
-    return dish_tail ? form_beams_batch<3, 16, true> : form_beams_batch<3, 16, false>;
+    return batch_instance<3, 16>(dish_tail, jl);
   case 4:
-    return dish_tail ? form_beams_batch<4, 16, true> : form_beams_batch<4, 16, false>;
+    return batch_instance<4, 16>(dish_tail, jl);
   default:
     return nullptr;
   }

@@ -4,23 +4,23 @@
 
 namespace bf {
 
-MfmaKernel mfma_kernel_ks64(int nt, bool dish_tail) {
+MfmaKernel mfma_kernel_ks64(int nt, bool dish_tail, int jl) {
   switch (nt) {
   case 1:
-    return dish_tail ? form_beams_mfma<1, 64, true> : form_beams_mfma<1, 64, false>;
+    return mfma_instance<1, 64>(dish_tail, jl);
   case 2:
-    return dish_tail ? form_beams_mfma<2, 64, true> : form_beams_mfma<2, 64, false>;
+    return mfma_instance<2, 64>(dish_tail, jl);
   default:
     return nullptr;
   }
 }
 
-BatchKernel batch_kernel_ks64(int nt, bool dish_tail) {
+BatchKernel batch_kernel_ks64(int nt, bool dish_tail, int jl) {
   switch (nt) {
   case 1:
-    return dish_tail ? form_beams_batch<1, 64, true> : form_beams_batch<1, 64, false>;
+    return batch_instance<1, 64>(dish_tail, jl);
   case 2:
-    return dish_tail ? form_beams_batch<2, 64, true> : form_beams_batch<2, 64, false>;
+    return batch_instance<2, 64>(dish_tail, jl);
   default:
     return nullptr;
   }

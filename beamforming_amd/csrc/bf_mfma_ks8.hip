@@ -4,31 +4,31 @@
 
 namespace bf {
 
-MfmaKernel mfma_kernel_ks8(int nt, bool dish_tail) {
+MfmaKernel mfma_kernel_ks8(int nt, bool dish_tail, int jl) {
   switch (nt) {
   case 1:
-    return dish_tail ? form_beams_mfma<1, 8, true> : form_beams_mfma<1, 8, false>;
+    return mfma_instance<1, 8>(dish_tail, jl);
   case 2:
-    return dish_tail ? form_beams_mfma<2, 8, true> : form_beams_mfma<2, 8, false>;
+    return mfma_instance<2, 8>(dish_tail, jl);
   case 3:
-    return dish_tail ? form_beams_mfma<3, 8, true> : form_beams_mfma<3, 8, false>;
+    return mfma_instance<3, 8>(dish_tail, jl);
   case 4:
-    return dish_tail ? form_beams_mfma<4, 8, true> : form_beams_mfma<4, 8, false>;
+    return mfma_instance<4, 8>(dish_tail, jl);
   default:
     return nullptr;
   }
 }
 
-BatchKernel batch_kernel_ks8(int nt, bool dish_tail) {
+BatchKernel batch_kernel_ks8(int nt, bool dish_tail, int jl) {
   switch (nt) {
   case 1:
-    return dish_tail ? form_beams_batch<1, 8, true> : form_beams_batch<1, 8, false>;
+    return batch_instance<1, 8>(dish_tail, jl);
   case 2:
-    return dish_tail ? form_beams_batch<2, 8, true> : form_beams_batch<2, 8, false>;
+    return batch_instance<2, 8>(dish_tail, jl);
   case 3:
-    return dish_tail ? form_beams_batch<3, 8, true> : form_beams_batch<3, 8, false>;
+    return batch_instance<3, 8>(dish_tail, jl);
   case 4:
-    return dish_tail ? form_beams_batch<4, 8, true> : form_beams_batch<4, 8, false>;
+    return batch_instance<4, 8>(dish_tail, jl);
   default:
     return nullptr;
   }
