@@ -21,7 +21,8 @@ Prints one JSON line (stdout) with:
                  that clock
   host_path      the copy-in / launch / copy-out entry point on pinned host
                  buffers (PCIe-inclusive, never the value)
-  other_configs  the small and wide configurations, timed the same way
+  other_configs  the small and wide configurations and the full-size shape
+                 with the J2 output layout, timed the same way
   batched_vs_plain_launch, serving, ingest   the batching, serving and E
                  ingest paths on their own workloads
 
@@ -333,12 +334,15 @@ def run_host_path(bf, torch, plan, dE, shape, reps=3):
 def run_other_configs(bf, torch, stream, steps, warmup):
     """The other single-GPU configurations, each timed like the full-size
     launch: small (T=256 F=4 D=64 B=16, seeded uniform E and A, bound by the
-    launch latency) and wide (T=8192 F=16 D=512 B=512, uniform E, phase-ramp
-    weights)."""
+    launch latency), wide (T=8192 F=16 D=512 B=512, uniform E, phase-ramp
+    weights), and the full-size shape with the J2 output layout
+    J[b][f][t][p]."""
     sptr = stream.cuda_stream
     out = {}
-    for name, (T, F, D, B), ramp in [("small", (256, 4, 64, 16), False), ("wide", (8192, 16, 512, 512), True)]:
-        shape = bf.Shape.make(T, F, D, B)
+    for name, (T, F, D, B), ramp, layout in [("small", (256, 4, 64, 16), False, 0),
+                                             ("wide", (8192, 16, 512, 512), True, 0),
+                                             ("full_j2_layout", (32768, 32, 512, 96), False, 1)]:
+        shape = bf.Shape.make(T, F, D, B, j_layout=layout)
         A = phase_ramp_weights(F, B, D) if ramp else host_uniform(bf, torch, F * B * D, 21)
         G = reference_gains(F * B, D)
         dE = torch.empty(shape.e_bytes, dtype=torch.uint8, device="cuda")
@@ -350,7 +354,8 @@ def run_other_configs(bf, torch, stream, steps, warmup):
             desc = plan.describe()
         tops = workload_ops(T, F, D, B) / (ms * 1e-3) / 1e12
         out[name] = {
-            "shape": {"ntimes": T, "nfrequencies": F, "ndishes": D, "npolarizations": 2, "nbeams": B},
+            "shape": {"ntimes": T, "nfrequencies": F, "ndishes": D, "npolarizations": 2, "nbeams": B,
+                      "j_layout": "[b][f][t][p]" if layout else "[b][f][p][t]"},
             "weights": "phase ramp" if ramp else "seeded uniform",
             "us_per_launch": round(ms * 1e3, 2),
             "tops": round(tops, 2),
