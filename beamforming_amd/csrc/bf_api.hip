@@ -232,6 +232,63 @@ int prepare_weights(bf_plan *p, hipStream_t st) {
   return BF_OK;
 }
 
+void destroy_host_path(HostPath *hp) {
+  if (!hp) return;
+  for (hipStream_t st : hp->st)
+    if (st) (void)hipStreamSynchronize(st);
+  if (hp->slabs) (void)hipFree(hp->slabs);
+  for (int k = 0; k < 2; ++k)
+    for (hipEvent_t ev : {hp->ev_in[k], hp->ev_run[k], hp->ev_out[k]})
+      if (ev) (void)hipEventDestroy(ev);
+  for (hipStream_t st : hp->st)
+    if (st) (void)hipStreamDestroy(st);
+  delete hp;
+}
+
+// The plan's host-path resources, made on first use: about 8 slabs of whole
+// 16-sample tiles (at least 256 samples) per call, two slab buffers in one
+// allocation (E and J of a slab each, 256-byte aligned).
+static std::mutex g_host_path_mutex;
+int host_path(bf_plan *p, HostPath **out) {
+  std::lock_guard<std::mutex> lock(g_host_path_mutex);
+  if (p->host) {
+    *out = p->host;
+    return BF_OK;
+  }
+  const bf_shape &s = p->shape;
+  const int64_t T = s.ntimes;
+  HostPath *hp = new HostPath();
+  hp->slab = T <= 256 ? T : std::max<int64_t>(256, ((T + 7) / 8 + 15) / 16 * 16);
+  hp->nslab = (T + hp->slab - 1) / hp->slab;
+  hp->nbuf = hp->nslab > 1 ? 2 : 1;
+  const size_t e_row = (size_t)(s.nfrequencies * s.ndishes * s.npolarizations);
+  const size_t j_row = (size_t)(s.nbeams * s.nfrequencies * s.npolarizations);
+  const size_t e_bytes = ((size_t)hp->slab * e_row + 255) / 256 * 256, j_bytes = ((size_t)hp->slab * j_row + 255) / 256 * 256;
+  hipError_t e = hipSuccess;
+  int prev = -1;
+  const bool switch_dev = hipGetDevice(&prev) == hipSuccess && prev != p->device;
+  if (switch_dev) e = hipSetDevice(p->device);
+  for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipStreamCreateWithFlags(&hp->st[k], hipStreamNonBlocking);
+  for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+    e = hipEventCreateWithFlags(&hp->ev_in[k], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&hp->ev_run[k], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&hp->ev_out[k], hipEventDisableTiming);
+  }
+  if (e == hipSuccess) e = hipMalloc(&hp->slabs, hp->nbuf * (e_bytes + j_bytes));
+  if (switch_dev) (void)hipSetDevice(prev);
+  if (e != hipSuccess) {
+    destroy_host_path(hp);
+    return hip_fail(e, "host path set-up");
+  }
+  for (int k = 0; k < hp->nbuf; ++k) {
+    hp->dE[k] = hp->slabs + k * (e_bytes + j_bytes);
+    hp->dJ[k] = hp->dE[k] + e_bytes;
+  }
+  p->host = hp;
+  *out = hp;
+  return BF_OK;
+}
+
 static void free_plan(bf_plan *p) {
   if (!p) return;
   if (p->W) (void)hipFree(p->W);
@@ -239,6 +296,7 @@ static void free_plan(bf_plan *p) {
   if (p->Gp) (void)hipFree(p->Gp);
   if (p->A_dev) (void)hipFree(p->A_dev);
   if (p->G_dev) (void)hipFree(p->G_dev);
+  destroy_host_path(p->host);
   delete p;
 }
 
@@ -373,9 +431,9 @@ bool pick_config(const bf_plan *p, int64_t T, int64_t F, MfmaConfig *c) {
 
 int launch_form_beams(const bf_plan *p, uint8_t *J, const uint8_t *E, hipStream_t st, bool allow_narrow) {
   const bf_shape &s = p->shape;
-  // The MFMA kernel reads E in 16-byte vectors and writes J in 4/8-byte words.
-  const bool aligned = ((uintptr_t)E & 15u) == 0 && ((uintptr_t)J & 7u) == 0;
-  if (p->use_mfma && aligned) {
+  // The MFMA kernel takes E and J at any byte address (unaligned vector
+  // accesses); the generic kernel serves only shapes and gains outside its range.
+  if (p->use_mfma) {
     if (allow_narrow && p->has_narrow) {
       MfmaConfig c;
       if (!pick_config(p, s.ntimes, s.nfrequencies, &c)) return fail(BF_ERR_UNSUPPORTED, "ntimes too large");
@@ -390,6 +448,56 @@ int launch_form_beams(const bf_plan *p, uint8_t *J, const uint8_t *E, hipStream_
     BF_HIP(hipGetLastError());
   }
   return BF_OK;
+}
+
+// Device Adler-32 of len bytes with `scratch` (kAdlerScratchWords words of
+// device memory): per-block partial pairs, their fold into the checksum in
+// the scratch's last words, then the 4-byte result copied to *out once the
+// stream has run it. Shared by bf_adler32_device (pooled scratch) and
+// bf_adler32_device_static (the library's static device scratch).
+int adler32_run(const uint8_t *data, size_t len, unsigned long long *scratch, uint32_t *out, hipStream_t st) {
+  const int blocks = adler32_blocks(data, (uint64_t)len);
+  uint32_t *result = reinterpret_cast<uint32_t *>(scratch + 2 * kAdlerBlocks);
+  hipLaunchKernelGGL(adler32_partial_kernel, dim3(blocks), dim3(kAdlerThreads), 0, st, data, (uint64_t)len, scratch);
+  BF_HIP(hipGetLastError());
+  hipLaunchKernelGGL(adler32_final_kernel, dim3(1), dim3(kAdlerThreads), 0, st, scratch, blocks, (uint64_t)len, result);
+  BF_HIP(hipGetLastError());
+  uint32_t h = 0;
+  hipError_t e = hipMemcpyAsync(&h, result, sizeof h, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "adler32 device reduction");
+  *out = h;
+  return BF_OK;
+}
+
+// Scratch of bf_adler32_device: a pool of device blocks per device, taken for
+// the length of one call and given back, so concurrent calls get distinct
+// blocks and steady use allocates nothing. Blocks are kept until the process
+// ends.
+static std::mutex g_adler_mutex;
+static std::vector<std::pair<int, unsigned long long *>> g_adler_pool; // (device, block)
+
+int adler_scratch_take(unsigned long long **out) {
+  int dev = 0;
+  BF_HIP(hipGetDevice(&dev));
+  {
+    std::lock_guard<std::mutex> lock(g_adler_mutex);
+    for (size_t i = 0; i < g_adler_pool.size(); ++i)
+      if (g_adler_pool[i].first == dev) {
+        *out = g_adler_pool[i].second;
+        g_adler_pool.erase(g_adler_pool.begin() + (std::ptrdiff_t)i);
+        return BF_OK;
+      }
+  }
+  BF_HIP(hipMalloc(out, kAdlerScratchWords * sizeof(unsigned long long)));
+  return BF_OK;
+}
+
+void adler_scratch_give(unsigned long long *p) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return; // dropped: leaked rather than reused on the wrong device
+  std::lock_guard<std::mutex> lock(g_adler_mutex);
+  g_adler_pool.emplace_back(dev, p);
 }
 
 } // namespace bf
@@ -486,73 +594,52 @@ int bf_form_beams_device(const bf_plan *p, uint8_t *J, const uint8_t *E, void *s
 // streams, so that E streams in back to back (the PCIe-bound part) while
 // earlier slabs are beamformed and their J goes out (full duplex). A slab's
 // J is [b][f][p][n] (or [b][f][n][p]); one 2-D copy puts its rows into the
-// columns t0 .. t0 + n - 1 of the caller's J.
+// columns t0 .. t0 + n - 1 of the caller's J. The streams, events and slab
+// buffers are made on the first call and kept with the plan.
 int bf_form_beams_host(const bf_plan *p, uint8_t *J, const uint8_t *E) {
   if (!p || !J || !E) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
   const bf_shape &s = p->shape;
   const int64_t T = s.ntimes;
   const size_t e_row = (size_t)(s.nfrequencies * s.ndishes * s.npolarizations); // E bytes per time sample
-  const size_t j_row = (size_t)(s.nbeams * s.nfrequencies * s.npolarizations);  // J bytes per time sample
-  // about 8 slabs, whole 16-sample tiles, at least 256 samples
-  const int64_t slab = T <= 256 ? T : std::max<int64_t>(256, ((T + 7) / 8 + 15) / 16 * 16);
-  const int64_t nslab = (T + slab - 1) / slab;
-  const int nbuf = nslab > 1 ? 2 : 1;
+  HostPath *hp = nullptr;
+  if (int rc = host_path(const_cast<bf_plan *>(p), &hp)) return rc;
+  std::lock_guard<std::mutex> lock(hp->mu);
   enum { IN, RUN, OUT };
-  hipStream_t st[3] = {nullptr, nullptr, nullptr};
-  // per slab buffer: copy in done, kernel done (E buffer free), copy out done (J buffer free)
-  hipEvent_t ev_in[2] = {nullptr, nullptr}, ev_run[2] = {nullptr, nullptr}, ev_out[2] = {nullptr, nullptr};
-  uint8_t *dE[2] = {nullptr, nullptr}, *dJ[2] = {nullptr, nullptr};
-  uint8_t *slabs = nullptr; // one allocation: E and J slab buffers, 256-byte aligned
-  const size_t e_bytes = ((size_t)slab * e_row + 255) / 256 * 256, j_bytes = ((size_t)slab * j_row + 255) / 256 * 256;
+  hipStream_t *st = hp->st;
   int rc = BF_OK;
-  hipError_t e = hipSuccess;
   auto check = [&](hipError_t err, const char *what) {
     if (err != hipSuccess && rc == BF_OK) rc = hip_fail(err, what);
     return rc == BF_OK;
   };
-  for (int k = 0; k < 3 && rc == BF_OK; ++k) check(hipStreamCreateWithFlags(&st[k], hipStreamNonBlocking), "hipStreamCreate");
-  for (int k = 0; k < nbuf && rc == BF_OK; ++k)
-    check(hipEventCreateWithFlags(&ev_in[k], hipEventDisableTiming), "hipEventCreate") &&
-        check(hipEventCreateWithFlags(&ev_run[k], hipEventDisableTiming), "hipEventCreate") &&
-        check(hipEventCreateWithFlags(&ev_out[k], hipEventDisableTiming), "hipEventCreate");
-  if (rc == BF_OK && check(hipMalloc(&slabs, nbuf * (e_bytes + j_bytes)), "hipMalloc slabs"))
-    for (int k = 0; k < nbuf; ++k) {
-      dE[k] = slabs + k * (e_bytes + j_bytes);
-      dJ[k] = dE[k] + e_bytes;
-    }
   const bool tp = s.j_layout == BF_J_BFTP; // J rows of n samples x P, else of n samples
   const size_t pitch = tp ? (size_t)T * s.npolarizations : (size_t)T;
   const size_t rows = tp ? (size_t)(s.nbeams * s.nfrequencies) : (size_t)(s.nbeams * s.nfrequencies * s.npolarizations);
-  for (int64_t i = 0; i < nslab && rc == BF_OK; ++i) {
+  const int nbuf = hp->nbuf;
+  for (int64_t i = 0; i < hp->nslab && rc == BF_OK; ++i) {
     const int k = (int)(i % nbuf);
-    const bool reuse = i >= nbuf; // slab i - 2 used these buffers
-    const int64_t t0 = i * slab, n = std::min(slab, T - t0);
-    if (reuse && !check(hipStreamWaitEvent(st[IN], ev_run[k], 0), "hipStreamWaitEvent")) break;
-    if (!check(hipMemcpyAsync(dE[k], E + (size_t)t0 * e_row, (size_t)n * e_row, hipMemcpyHostToDevice, st[IN]),
+    const bool reuse = i >= nbuf; // slab i - 2 used these buffers (an earlier call has finished with them)
+    const int64_t t0 = i * hp->slab, n = std::min(hp->slab, T - t0);
+    if (reuse && !check(hipStreamWaitEvent(st[IN], hp->ev_run[k], 0), "hipStreamWaitEvent")) break;
+    if (!check(hipMemcpyAsync(hp->dE[k], E + (size_t)t0 * e_row, (size_t)n * e_row, hipMemcpyHostToDevice, st[IN]),
                "copy E slab") ||
-        !check(hipEventRecord(ev_in[k], st[IN]), "hipEventRecord") ||
-        !check(hipStreamWaitEvent(st[RUN], ev_in[k], 0), "hipStreamWaitEvent") ||
-        (reuse && !check(hipStreamWaitEvent(st[RUN], ev_out[k], 0), "hipStreamWaitEvent")))
+        !check(hipEventRecord(hp->ev_in[k], st[IN]), "hipEventRecord") ||
+        !check(hipStreamWaitEvent(st[RUN], hp->ev_in[k], 0), "hipStreamWaitEvent") ||
+        (reuse && !check(hipStreamWaitEvent(st[RUN], hp->ev_out[k], 0), "hipStreamWaitEvent")))
       break;
-    if ((rc = bf_form_beams_channels(p, 0, (int)s.nfrequencies, n, dJ[k], dE[k], st[RUN])) != BF_OK) break;
+    if ((rc = bf_form_beams_channels(p, 0, (int)s.nfrequencies, n, hp->dJ[k], hp->dE[k], st[RUN])) != BF_OK) break;
     const size_t width = tp ? (size_t)n * s.npolarizations : (size_t)n;
     const size_t col = tp ? (size_t)t0 * s.npolarizations : (size_t)t0;
-    if (!check(hipEventRecord(ev_run[k], st[RUN]), "hipEventRecord") ||
-        !check(hipStreamWaitEvent(st[OUT], ev_run[k], 0), "hipStreamWaitEvent") ||
-        !check(hipMemcpy2DAsync(J + col, pitch, dJ[k], width, width, rows, hipMemcpyDeviceToHost, st[OUT]),
+    if (!check(hipEventRecord(hp->ev_run[k], st[RUN]), "hipEventRecord") ||
+        !check(hipStreamWaitEvent(st[OUT], hp->ev_run[k], 0), "hipStreamWaitEvent") ||
+        !check(hipMemcpy2DAsync(J + col, pitch, hp->dJ[k], width, width, rows, hipMemcpyDeviceToHost, st[OUT]),
                "copy J slab") ||
-        !check(hipEventRecord(ev_out[k], st[OUT]), "hipEventRecord"))
+        !check(hipEventRecord(hp->ev_out[k], st[OUT]), "hipEventRecord"))
       break;
   }
-  for (int k = 0; k < 3; ++k)
-    if (st[k] && (e = hipStreamSynchronize(st[k])) != hipSuccess && rc == BF_OK) rc = hip_fail(e, "host path");
-  if (slabs) (void)hipFree(slabs);
-  for (int k = 0; k < nbuf; ++k) {
-    for (hipEvent_t ev : {ev_in[k], ev_run[k], ev_out[k]})
-      if (ev) (void)hipEventDestroy(ev);
+  for (int k = 0; k < 3; ++k) {
+    const hipError_t e = hipStreamSynchronize(st[k]);
+    if (e != hipSuccess && rc == BF_OK) rc = hip_fail(e, "host path");
   }
-  for (int k = 0; k < 3; ++k)
-    if (st[k]) (void)hipStreamDestroy(st[k]);
   return rc;
 }
 
@@ -586,22 +673,11 @@ uint32_t bf_adler32(const uint8_t *data, size_t len) {
 int bf_adler32_device(const uint8_t *data, size_t len, uint32_t *out, void *stream) {
   if (!data || !out) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  unsigned long long *acc = nullptr;
-  BF_HIP(hipMalloc(&acc, 2 * sizeof(unsigned long long)));
-  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, acc, 2);
-  const uint64_t nvec = len / 16;
-  const int blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>((nvec + 255) / 256, 4096));
-  hipLaunchKernelGGL(adler32_partial_kernel, dim3(blocks), dim3(256), 0, st, data, (uint64_t)len, acc);
-  unsigned long long h[2] = {0, 0};
-  hipError_t e = hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  (void)hipFree(acc);
-  if (e != hipSuccess) return hip_fail(e, "adler32 device reduction");
-  const uint64_t mod = 65521u;
-  const uint64_t a = (1 + h[0]) % mod;
-  const uint64_t b = ((uint64_t)(len % mod) + h[1]) % mod;
-  *out = (uint32_t)((b << 16) | a);
-  return BF_OK;
+  unsigned long long *scratch = nullptr;
+  if (int rc = adler_scratch_take(&scratch)) return rc;
+  const int rc = adler32_run(data, len, scratch, out, st);
+  adler_scratch_give(scratch);
+  return rc;
 }
 
 int bf_fill_uniform_device(uint8_t *buf, size_t n, uint64_t seed, uint64_t offset, void *stream) {

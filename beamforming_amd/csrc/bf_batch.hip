@@ -167,8 +167,6 @@ int prepare(bf_batch *b, const bf_chunk *chunks, int n, int *block) {
         k.channel + nf > p->shape.nfrequencies)
       return fail(BF_ERR_INVALID_ARGUMENT, "bad chunk " + std::to_string(i) +
                                                " (NULL buffer, ntimes < 1, or channels outside the table)");
-    if (((uintptr_t)k.E & 15u) || ((uintptr_t)k.J & 7u))
-      return fail(BF_ERR_INVALID_ARGUMENT, "chunk " + std::to_string(i) + ": E must be 16-byte and J 8-byte aligned");
   }
   for (int i = 0; i < n; ++i)
     if (chunks[i].ntimes > kMaxTimes) return fail(BF_ERR_UNSUPPORTED, "chunk too long");
@@ -289,8 +287,7 @@ int bf_form_beams_channels(const bf_plan *p, int channel, int nfreq, int64_t nti
   if (int rc = check_channel_args(p, channel, nfreq, ntimes, J, E)) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const bf_shape &s = p->shape;
-  const bool aligned = ((uintptr_t)E & 15u) == 0 && ((uintptr_t)J & 7u) == 0;
-  if (p->use_mfma && aligned) {
+  if (p->use_mfma) { // any alignment of E and J
     MfmaConfig c;
     if (!pick_config(p, ntimes, nfreq, &c)) return fail(BF_ERR_UNSUPPORTED, "ntimes too large");
     return launch_mfma(p, c, J, E, ntimes, nfreq, channel, st);

@@ -186,6 +186,37 @@ struct MfmaParams {
   int32_t stage_j;   // 1: J rows go through the LDS staging area (allocated in the launch's LDS)
 };
 
+// ---------------------------------------------------------------------------
+// Address arithmetic of the tile pipeline, shared with the host bounds check
+// (tests/bounds_check.cpp), which runs these same expressions for every wave
+// and lane over random shapes and chunk lists.
+//
+// E: lane (row, piece) of a tile loads 16 bytes of time row `row` (0..15),
+// bytes 16 piece .. + 15 of the row's 64-byte window of K-step pair j
+// (dishes 32 j .. 32 j + 31, both polarizations). Rows past T re-read row
+// T - 1 (their outputs are never stored).
+BF_HD int64_t e_row_index(int64_t tile, int row, int64_t T) {
+  const int64_t t = tile * kTileT + row;
+  return t < T ? t : T - 1;
+}
+// Byte offset within the row of the piece's load, relative to byte
+// 16 piece. With a dish tail (dt), pieces wholly past D re-read the row's
+// first 16 bytes (their weights are zero).
+BF_HD int64_t e_col_offset(int j, int piece, int D, bool dt) {
+  if (dt) return 32 * j + 8 * piece < D ? 64 * j : -16 * piece;
+  return 64 * j;
+}
+// First byte of J row (beam b, frequency f, polarization p) of
+// J[b][F][p][T] ([b][f][p][t]) and of beam row (b, f) of J[b][F][T][2]
+// ([b][f][t][p]).
+BF_HD int64_t j_row_bfpt(int b, int F, int f, int p, int64_t T) { return (((int64_t)b * F + f) * 2 + p) * T; }
+BF_HD int64_t j_row_bftp(int b, int F, int f, int64_t T) { return ((int64_t)b * F + f) * T * 2; }
+// Tiles [begin, end) of time chunk `chunk` of a plain launch.
+BF_HD void chunk_tiles(int chunk, int tiles_per_chunk, int64_t ntiles, int64_t *begin, int64_t *end) {
+  *begin = (int64_t)chunk * tiles_per_chunk;
+  *end = ntiles < *begin + tiles_per_chunk ? ntiles : *begin + tiles_per_chunk;
+}
+
 // Work index of a workgroup. Workgroups are dispatched round-robin over the
 // 8 XCDs; this gives each XCD a contiguous range of work items, so the beam
 // groups of one time chunk (which read the same E rows) and consecutive

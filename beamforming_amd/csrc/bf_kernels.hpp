@@ -12,8 +12,18 @@ __global__ void prep_corr_kernel(const uint8_t *__restrict__ A, const float *__r
 __global__ void form_beams_generic_kernel(const uint8_t *__restrict__ E, uint8_t *__restrict__ J,
                                           const uint8_t *__restrict__ A, const float *__restrict__ G, int64_t T,
                                           int F, int D, int P, int B, int jlayout);
-__global__ void adler32_partial_kernel(const uint8_t *__restrict__ data, uint64_t n, unsigned long long *acc);
-__global__ void zero_u64_kernel(unsigned long long *p, int n);
+// Device Adler-32: kAdlerThreads threads per block, at most kAdlerBlocks
+// blocks; the scratch of one checksum is kAdlerScratchWords 64-bit words
+// (2 per block, then the 32-bit result).
+constexpr uint32_t kAdlerMod = 65521u;
+constexpr int kAdlerThreads = 256;
+constexpr int kAdlerBlocks = 2048;
+constexpr int kAdlerScratchWords = 2 * kAdlerBlocks + 2;
+__global__ void adler32_partial_kernel(const uint8_t *__restrict__ data, uint64_t n,
+                                       unsigned long long *__restrict__ partials);
+__global__ void adler32_final_kernel(const unsigned long long *__restrict__ partials, int nblocks, uint64_t n,
+                                     uint32_t *__restrict__ out);
+int adler32_blocks(const uint8_t *data, uint64_t n);
 __global__ void fill_uniform_kernel(uint8_t *__restrict__ buf, uint64_t n, uint64_t seed, uint64_t offset);
 
 // MFMA kernel instances, one translation unit per K-step count.

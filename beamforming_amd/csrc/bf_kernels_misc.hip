@@ -91,11 +91,12 @@ __global__ void form_beams_generic_kernel(const uint8_t *__restrict__ E, uint8_t
 // Adler-32 (adler32.c:7-20) in closed form: with bytes x_i, i = 0..n-1,
 //   a = 1 + sum x_i,   b = n + sum (n - i) x_i   (mod 65521).
 // Each thread folds 16-byte vectors into (sum x, sum (n - i) x) mod 65521;
-// block sums go to two 64-bit counters (each term < 65521, so they cannot
-// overflow for any realistic grid).
-constexpr uint32_t kAdlerMod = 65521u;
+// each block writes its pair, reduced mod 65521, to partials[2 block ..];
+// adler32_final_kernel folds the pairs (in block order) into the checksum.
+// No atomics and no zeroing: every pair is written by its block.
 
-__global__ void adler32_partial_kernel(const uint8_t *__restrict__ data, uint64_t n, unsigned long long *acc) {
+__global__ void adler32_partial_kernel(const uint8_t *__restrict__ data, uint64_t n,
+                                       unsigned long long *__restrict__ partials) {
   // Bytes before the first 16-byte boundary and after the last full vector
   // are folded in by thread 0 of block 0.
   const uint64_t pre = ((16u - ((uintptr_t)data & 15u)) & 15u) < n ? ((16u - ((uintptr_t)data & 15u)) & 15u) : n;
@@ -133,19 +134,63 @@ __global__ void adler32_partial_kernel(const uint8_t *__restrict__ data, uint64_
   }
   sa %= kAdlerMod;
   sb %= kAdlerMod;
-  for (int off = 32; off > 0; off >>= 1) {
+  for (int off = 32; off > 0; off >>= 1) { // each term < 65521: no overflow
     sa += __shfl_down(sa, off, 64);
     sb += __shfl_down(sb, off, 64);
   }
+  __shared__ uint64_t wsum[2][kAdlerThreads / 64];
   if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&acc[0], (unsigned long long)sa);
-    atomicAdd(&acc[1], (unsigned long long)sb);
+    wsum[0][threadIdx.x >> 6] = sa;
+    wsum[1][threadIdx.x >> 6] = sb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t a = 0, b = 0;
+    for (int w = 0; w < kAdlerThreads / 64; ++w) {
+      a += wsum[0][w];
+      b += wsum[1][w];
+    }
+    partials[2 * blockIdx.x + 0] = a % kAdlerMod;
+    partials[2 * blockIdx.x + 1] = b % kAdlerMod;
   }
 }
 
-__global__ void zero_u64_kernel(unsigned long long *p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
+// One block: folds nblocks pairs (each < 65521) into the checksum of n bytes.
+__global__ void adler32_final_kernel(const unsigned long long *__restrict__ partials, int nblocks, uint64_t n,
+                                     uint32_t *__restrict__ out) {
+  uint64_t a = 0, b = 0;
+  for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
+    a += partials[2 * i + 0];
+    b += partials[2 * i + 1];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+  }
+  __shared__ uint64_t wsum[2][kAdlerThreads / 64];
+  if ((threadIdx.x & 63) == 0) {
+    wsum[0][threadIdx.x >> 6] = a;
+    wsum[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t ta = 0, tb = 0;
+    for (int w = 0; w < kAdlerThreads / 64; ++w) {
+      ta += wsum[0][w];
+      tb += wsum[1][w];
+    }
+    const uint64_t ra = (1 + ta) % kAdlerMod;
+    const uint64_t rb = (n % kAdlerMod + tb) % kAdlerMod;
+    out[0] = (uint32_t)((rb << 16) | ra);
+  }
+}
+
+// Blocks of the partial pass for n bytes at data (at most kAdlerBlocks).
+int adler32_blocks(const uint8_t *data, uint64_t n) {
+  const uint64_t pre = (16u - ((uintptr_t)data & 15u)) & 15u;
+  const uint64_t nvec = n > pre ? (n - pre) / 16 : 0;
+  const uint64_t b = (nvec + kAdlerThreads - 1) / kAdlerThreads;
+  return (int)(b < 1 ? 1 : (b > (uint64_t)kAdlerBlocks ? (uint64_t)kAdlerBlocks : b));
 }
 
 // ---------------------------------------------------------------------------

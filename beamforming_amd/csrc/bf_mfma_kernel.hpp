@@ -56,8 +56,10 @@ template <class T> __device__ __forceinline__ const BF_GLOBAL T *gload_ptr(const
 template <class T> __device__ __forceinline__ BF_GLOBAL T *gstore_ptr(void *p) { return (BF_GLOBAL T *)p; }
 
 // Vector types without an alignment promise: gfx950 runs in unaligned
-// access mode, so a 16- or 8-byte store to any address is one instruction
-// (split by the memory system where it crosses a line).
+// access mode, so a 16- or 8-byte load or store at any address is one
+// instruction (split by the memory system where it crosses a line). E loads
+// and J stores use them, so E and J may sit at any byte address; the code is
+// the same as for aligned types.
 typedef v4i v4i_u __attribute__((aligned(1)));
 typedef int v2i __attribute__((ext_vector_type(2)));
 typedef v2i v2i_u __attribute__((aligned(1)));
@@ -188,14 +190,9 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
   const int ld_row = lane >> 2, ld_piece = lane & 3;
   const int64_t e_tstride = job.e_tstride;
   const uint8_t *Ef = job.E + ld_piece * 16;
-  const int64_t tlast = job.T - 1;
-  auto row_ptr = [&](int64_t tl) -> const uint8_t * {
-    const int64_t t = tl * kTileT + ld_row;
-    return Ef + (t < job.T ? t : tlast) * e_tstride;
-  };
+  auto row_ptr = [&](int64_t tl) -> const uint8_t * { return Ef + e_row_index(tl, ld_row, job.T) * e_tstride; };
   auto load_pair = [&](const uint8_t *p, int j) -> v4i {
-    if (DT) return *gload_ptr<v4i>(p + (32 * j + 8 * ld_piece < job.D ? 64 * j : -16 * ld_piece));
-    return *gload_ptr<v4i>(p + 64 * j);
+    return *gload_ptr<v4i_u>(p + e_col_offset(j, ld_piece, job.D, DT)); // any alignment
   };
   const int wr_slot = ld_row * 4 + ((ld_piece + (ld_row >> 1)) & 3);
   int rd_slot[2];
@@ -335,7 +332,7 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
         for (int nt = 0; nt < NT; ++nt) {
           const int b = (job.bg * NT + nt) * 32 + col;
           if (b < job.B)
-            store_piece(job.J + (((int64_t)b * job.F + job.f) * 2 + h) * job.T + t0, row[nt], nvalid, fast_store);
+            store_piece(job.J + j_row_bfpt(b, job.F, job.f, h, job.T) + t0, row[nt], nvalid, fast_store);
         }
       }
     } else if (Sl) {
@@ -365,7 +362,7 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
         const int b = (job.bg * NT + nt) * 32 + col;
         if (b < job.B) {
           // J[b][f][t][p]: interleave the polarizations of 4 times per half.
-          uint8_t *row = job.J + (((int64_t)b * job.F + job.f) * job.T + t0) * 2;
+          uint8_t *row = job.J + j_row_bftp(b, job.F, job.f, job.T) + 2 * t0;
 #pragma unroll
           for (int k = 0; k < 2; ++k) {
             const int tt = 8 * k + 4 * h; // first time of this group in the tile
@@ -399,7 +396,7 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
         const int b = job.bg * NT * 32 + (srow >> 1);
         const int64_t t = t_round + 16 * seg;
         if (b < job.B && t < t_end)
-          store_piece(job.J + (((int64_t)b * job.F + job.f) * 2 + (srow & 1)) * job.T + t, v,
+          store_piece(job.J + j_row_bfpt(b, job.F, job.f, srow & 1, job.T) + t, v,
                       t_end - t < 16 ? (int)(t_end - t) : 16, fast_store);
       }
     } else {
@@ -413,7 +410,7 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
         const int b = job.bg * NT * 32 + srow;
         const int64_t t = t_round + 8 * seg;
         if (b < job.B && t < t_end)
-          store_piece(job.J + (((int64_t)b * job.F + job.f) * job.T + t) * 2, v,
+          store_piece(job.J + j_row_bftp(b, job.F, job.f, job.T) + 2 * t, v,
                       t_end - t < 8 ? (int)(2 * (t_end - t)) : 16, fast_store);
       }
     }
@@ -436,8 +433,8 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
 
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t ntiles = (a.T + kTileT - 1) / kTileT;
-  const int64_t tile_begin = (int64_t)wi.chunk * a.tiles_per_chunk;
-  const int64_t tile_end = ntiles < tile_begin + a.tiles_per_chunk ? ntiles : tile_begin + a.tiles_per_chunk;
+  int64_t tile_begin, tile_end;
+  chunk_tiles(wi.chunk, a.tiles_per_chunk, ntiles, &tile_begin, &tile_end);
   TileJob job;
   job.E = a.E + (int64_t)wi.f * a.D * 2;
   job.e_tstride = (int64_t)a.F * a.D * 2;

@@ -4,6 +4,7 @@
 #include "beamform.h"
 #include "bf_kernels.hpp"
 
+#include <mutex>
 #include <string>
 
 namespace bf {
@@ -34,6 +35,20 @@ struct TuneChoice {
   bool stage_j = true; // stage J rows in LDS when the layout and the LDS allow it
 };
 
+// Resources of the host-buffer entry point (bf_form_beams_host), made on
+// its first call and kept with the plan: the copy-in, kernel and copy-out
+// streams, the events ordering slab-buffer reuse, and the two slab buffers
+// (one device allocation). Calls on one plan are serialised by `mu`.
+struct HostPath {
+  std::mutex mu;
+  hipStream_t st[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_in[2] = {nullptr, nullptr}, ev_run[2] = {nullptr, nullptr}, ev_out[2] = {nullptr, nullptr};
+  uint8_t *slabs = nullptr;
+  uint8_t *dE[2] = {nullptr, nullptr}, *dJ[2] = {nullptr, nullptr};
+  int64_t slab = 0, nslab = 0; // samples per slab, slabs per call
+  int nbuf = 0;
+};
+
 } // namespace bf
 
 struct bf_plan {
@@ -57,6 +72,7 @@ struct bf_plan {
   bf::v4i *W = nullptr;     // [F][KS][NTP][64] weight operands
   int *corr = nullptr;      // [F][BP][2]
   float *Gp = nullptr;      // [F][BP]
+  bf::HostPath *host = nullptr; // bf_form_beams_host resources (made on first use)
 };
 
 namespace bf {

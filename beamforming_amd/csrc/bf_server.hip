@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -61,7 +62,13 @@ struct Request {
 struct Launch {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
+  // busy: the slot is taken (from the moment a thread picks it until its
+  // requests are finished). issued: the launch's copies, kernel and `done`
+  // event have been enqueued; set under the lock once issue() has returned.
+  // Until then hipEventQuery(done) reports an earlier record (or none, which
+  // also reads as complete), so the completion scan must not look at it.
   bool busy = false;
+  bool issued = false;
   std::vector<uint64_t> tickets;
 };
 
@@ -106,6 +113,12 @@ struct bf_server {
   bool stopping = false;
   std::thread loop;
   std::string loop_error;
+
+  // Pause between taking a launch slot and enqueueing the launch, from the
+  // environment variable BF_SERVER_ISSUE_DELAY_US (0 when unset): widens the
+  // window in which a launch is taken but not yet enqueued, so that tests can
+  // show that the completion scan leaves such launches alone.
+  int issue_delay_us = 0;
 
   uint64_t n_submitted = 0, n_admitted = 0, n_completed = 0, n_released = 0, n_rejected = 0, n_launches = 0;
   uint64_t n_failed = 0, max_batch_seen = 0, batch_requests = 0;
@@ -169,6 +182,7 @@ void finish(bf_server *s, Launch &L, int rc, double t) {
   }
   L.tickets.clear();
   L.busy = false;
+  L.issued = false;
 }
 
 void admit_locked(bf_server *s) {
@@ -221,13 +235,16 @@ bool launch_locked(bf_server *s, std::unique_lock<std::mutex> &lk) {
   }
   for (uint64_t id : L.tickets) s->reqs[id].batch_size = (int)L.tickets.size();
   L.busy = true; // the slot is ours while the lock is dropped
+  L.issued = false;
   ++s->n_launches;
   s->batch_requests += L.tickets.size();
   s->max_batch_seen = std::max<uint64_t>(s->max_batch_seen, L.tickets.size());
   lk.unlock();
+  if (s->issue_delay_us > 0) std::this_thread::sleep_for(std::chrono::microseconds(s->issue_delay_us));
   const int rc = issue(s, L, plan, batch, chunks, copies); // Request pointers stay valid: entries are only erased on release
   if (rc) (void)hipStreamSynchronize(L.stream); // no staging copy of a failed launch may still write the arena
   lk.lock();
+  L.issued = true; // from here on `done` belongs to this launch
   if (rc) {
     s->loop_error = g_last_error;
     finish(s, L, rc, s->now());
@@ -252,6 +269,10 @@ void run_loop(bf_server *s) {
     bool any_busy = false;
     for (Launch &L : s->launches) {
       if (!L.busy) continue;
+      if (!L.issued) { // another thread is still enqueueing this launch
+        any_busy = true;
+        continue;
+      }
       const hipError_t q = hipEventQuery(L.done);
       if (q == hipErrorNotReady) {
         any_busy = true;
@@ -324,6 +345,7 @@ int bf_server_create(const bf_plan *p, const bf_server_config *cfg, bf_server **
   if (!s->cfg.max_launches) s->cfg.max_launches = 2;
   s->device = p->device;
   s->epoch = std::chrono::steady_clock::now();
+  if (const char *d = std::getenv("BF_SERVER_ISSUE_DELAY_US")) s->issue_delay_us = std::max(0, std::min(1000000, std::atoi(d)));
   auto bail = [&](int rc) {
     bf_server_destroy(s);
     return rc;
@@ -389,8 +411,6 @@ int bf_server_submit_many(bf_server *s, const bf_request *reqs, int n, uint64_t 
         (req->e_on_host != 0 && req->e_on_host != 1))
       return fail(BF_ERR_INVALID_ARGUMENT, "bad request " + std::to_string(i) +
                                                " (NULL E, ntimes < 1, channels outside the table, flags)");
-    if (!req->e_on_host && ((uintptr_t)req->E & 15u))
-      return fail(BF_ERR_INVALID_ARGUMENT, "request " + std::to_string(i) + ": device E must be 16-byte aligned");
     Request &r = rs[i];
     r.tag = req->tag;
     r.E_src = req->E;

@@ -250,6 +250,30 @@ def power_sample(fn, seconds):
     }
 
 
+def batched_chunk_lengths(C=64, seed=7):
+    """ntimes of the batched workload's C channels: log-uniform in
+    [512, 32768], rounded to integers, drawn from numpy's PCG64 with `seed`."""
+    rng = np.random.default_rng(seed)
+    return [int(x) for x in np.rint(np.exp(rng.uniform(np.log(512), np.log(32768), C)))]
+
+
+def serving_requests(n, rate, rng, C, D, pool_bytes):
+    """The serving workload's n requests, drawn from `rng` (a numpy
+    Generator): Poisson arrival times (exponential gaps of mean 1 / rate, the
+    first at 0), ntimes uniform in [1024, 16384], nfreq uniform in {1, 4, 16},
+    a first channel uniform among those that fit in the C-channel table, and
+    the 256-byte aligned offset of the request's E inside a pool of
+    pool_bytes. Returns (arrivals, ntimes, nfreq, channel, offset) arrays."""
+    gaps = rng.exponential(1.0 / rate, n)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    Ts = rng.integers(1024, 16385, n)
+    nfs = rng.choice([1, 4, 16], n)
+    chs = np.array([rng.integers(0, C - nf + 1) for nf in nfs])
+    ebytes = Ts * nfs * D * 2
+    offs = np.array([rng.integers(0, (pool_bytes - e) // 256 + 1) * 256 for e in ebytes])
+    return arrivals, Ts, nfs, chs, offs
+
+
 def aligned_offsets(sizes, align=256):
     offs, total = [], 0
     for n in sizes:
@@ -263,8 +287,7 @@ def run_batched(bf, torch, stream, steps, warmup, seed=7, C=64, D=512, B=96):
     one persistent launch (plain and hipGraph replay) against one
     form_beams launch per chunk on the same stream."""
     sptr = stream.cuda_stream
-    rng = np.random.default_rng(seed)
-    Ts = [int(x) for x in np.rint(np.exp(rng.uniform(np.log(512), np.log(32768), C)))]
+    Ts = batched_chunk_lengths(C, seed)
     A = host_uniform(bf, torch, C * B * D, seed + 1)
     G = reference_gains(C * B, D)
     e_off, e_total = aligned_offsets([T * D * 2 for T in Ts])
@@ -308,7 +331,10 @@ def run_batched(bf, torch, stream, steps, warmup, seed=7, C=64, D=512, B=96):
     }
     batch.close()
     plan.close()
-    return res
+    # inputs and outputs for the oracle check in the CPU leg (every chunk)
+    check = {"Ts": Ts, "A": A, "G": G, "D": D, "B": B, "C": C, "e_off": e_off, "j_off": j_off,
+             "E": dE.cpu().numpy(), "J": dJb.cpu().numpy()}
+    return res, check
 
 
 def run_host_path(bf, torch, plan, dE, shape, reps=3):
@@ -352,6 +378,9 @@ def run_other_configs(bf, torch, stream, steps, warmup):
             ms = timed(stream, steps, warmup, lambda: plan.form_beams_device(dJ.data_ptr(), dE.data_ptr(), sptr),
                        settle_s=0.2)
             desc = plan.describe()
+        if name == "small":
+            small_check = {"T": T, "F": F, "D": D, "B": B, "E": dE.cpu().numpy(), "A": A, "G": G,
+                           "J": dJ.cpu().numpy(), "us_per_launch": ms * 1e3}
         tops = workload_ops(T, F, D, B) / (ms * 1e-3) / 1e12
         out[name] = {
             "shape": {"ntimes": T, "nfrequencies": F, "ndishes": D, "npolarizations": 2, "nbeams": B,
@@ -364,7 +393,7 @@ def run_other_configs(bf, torch, stream, steps, warmup):
             "beam_samples_per_s": B * F * T * 2 / (ms * 1e-3),
             "kernel": desc,
         }
-    return out
+    return out, small_check
 
 
 def run_ingest(bf, torch, stream, dE, A, G, T, F, D, B, packet=256, TC=8192, groups=4, slots=4):
@@ -457,17 +486,16 @@ def run_serving(bf, torch, stream, seconds=2.0, rate=2000.0, seed=9, C=64, D=512
     bf.fill_uniform_device(pool.data_ptr(), pool_bytes, seed + 2, 0, sptr)
     torch.cuda.synchronize()
     n = int(seconds * rate)
-    gaps = rng.exponential(1.0 / rate, n)
-    arrivals = np.cumsum(gaps) - gaps[0]
-    Ts = rng.integers(1024, 16385, n)
-    nfs = rng.choice([1, 4, 16], n)
-    chs = np.array([rng.integers(0, C - nf + 1) for nf in nfs])
-    ebytes = Ts * nfs * D * 2
-    offs = np.array([rng.integers(0, (pool_bytes - e) // 256 + 1) * 256 for e in ebytes])
+    arrivals, Ts, nfs, chs, offs = serving_requests(n, rate, rng, C, D, pool_bytes)
     reqs = [(pool.data_ptr() + int(offs[i]), int(Ts[i]), int(chs[i]), int(nfs[i]), i) for i in range(n)]
     plan = bf.Plan(bf.Shape.make(1, C, D, B), A, G)
     srv = bf.Server(plan, hbm_budget=budget)
     results = {}
+    # J of a seeded sample of the Poisson phase's requests, kept for the
+    # oracle check in the CPU leg (read after completion, before release)
+    sample = set(range(0, n, max(1, n // 16)))
+    kept = {}
+    keep = [False]
 
     def consume(count):
         while len(results) < count:
@@ -477,6 +505,8 @@ def run_serving(bf, torch, stream, seconds=2.0, rate=2000.0, seed=9, C=64, D=512
             if r.status != 0:
                 raise RuntimeError(f"request {r.tag} failed: status {r.status}")
             results[r.tag] = (r.t_submit, r.t_done, r.batch_size)
+            if keep[0] and r.tag in sample:
+                kept[r.tag] = srv.read(r.ticket, r.j_bytes)
             srv.release(r.ticket)
 
     # warm-up (graph-free path, kernels loaded, arena touched)
@@ -484,6 +514,7 @@ def run_serving(bf, torch, stream, seconds=2.0, rate=2000.0, seed=9, C=64, D=512
     consume(32)
     results.clear()
 
+    keep[0] = True
     th = threading.Thread(target=consume, args=(n,))
     th.start()
     t0 = time.perf_counter()
@@ -499,6 +530,7 @@ def run_serving(bf, torch, stream, seconds=2.0, rate=2000.0, seed=9, C=64, D=512
         late.append(time.perf_counter() - due)
         srv.submit(q[0], q[1], q[2], q[3], tag=q[4])
     th.join()
+    keep[0] = False
     lat = np.array([results[i][1] - results[i][0] for i in range(n)])
     t_first = min(v[0] for v in results.values())
     t_last = max(v[1] for v in results.values())
@@ -523,7 +555,11 @@ def run_serving(bf, torch, stream, seconds=2.0, rate=2000.0, seed=9, C=64, D=512
     plain_s = time.perf_counter() - tp
     srv.close()
     plan.close()
-    return {
+    check = {"A": A, "G": G, "C": C, "D": D, "B": B,
+             "requests": [(int(Ts[i]), int(chs[i]), int(nfs[i]),
+                           pool[int(offs[i]):int(offs[i]) + int(Ts[i] * nfs[i]) * D * 2].cpu().numpy(), kept[i])
+                          for i in sorted(kept)]}
+    res = {
         "workload": f"serving: Poisson arrivals {rate:.0f} chunks/s for {seconds:.1f} s (seed {seed}), nfreq in "
                     f"{{1,4,16}}, ntimes uniform [1024, 16384], D={D}, B={B}, P=2, {C}-channel table resident, "
                     f"E resident in HBM, HBM budget {budget >> 20} MiB",
@@ -542,6 +578,74 @@ def run_serving(bf, torch, stream, seconds=2.0, rate=2000.0, seed=9, C=64, D=512
         "plain_launch_beam_samples_per_s": sat_bs / plain_s,
         "saturation_speedup_vs_plain_launch": round(plain_s / sat_s, 3),
     }
+    return res, check
+
+
+def oracle_checks(batched_check, serving_check, small_check, threads):
+    """The CPU leg's checks of the other workloads' outputs with the oracle
+    (the CPU restatement of cpu.cxx, run per chunk or request alone): every
+    chunk of the batched workload, the sampled requests of the serving run,
+    and the small configuration, whose oracle run is also timed (median of
+    101 runs beside the GPU launch time)."""
+    from oracle import bf_oracle as bo
+    out = {}
+    if batched_check:
+        w = batched_check
+        t0 = time.perf_counter()
+        ok = True
+        for c, T in enumerate(w["Ts"]):
+            Ac = w["A"].reshape(w["C"], -1)[c]
+            Gc = w["G"].reshape(w["C"], -1)[c]
+            E = w["E"][w["e_off"][c]:w["e_off"][c] + T * w["D"] * 2]
+            want = bo.form_beams(E, Ac, Gc, T, 1, w["D"], w["B"], nthreads=threads)
+            ok = ok and bool(np.array_equal(w["J"][w["j_off"][c]:w["j_off"][c] + w["B"] * 2 * T], want))
+        out["batched"] = {"bit_exact_vs_oracle": ok, "chunks_checked": len(w["Ts"]),
+                          "oracle_seconds": round(time.perf_counter() - t0, 2)}
+    if serving_check:
+        w = serving_check
+        t0 = time.perf_counter()
+        ok = True
+        for T, c, nf, E, J in w["requests"]:
+            Ac = w["A"].reshape(w["C"], -1)[c:c + nf]
+            Gc = w["G"].reshape(w["C"], -1)[c:c + nf]
+            ok = ok and bool(np.array_equal(J, bo.form_beams(E, Ac, Gc, T, nf, w["D"], w["B"], nthreads=threads)))
+        out["serving"] = {"bit_exact_vs_oracle": ok, "requests_checked": len(w["requests"]),
+                          "sample": "every (n // 16)-th request of the Poisson phase, all of its outputs",
+                          "oracle_seconds": round(time.perf_counter() - t0, 2)}
+    if small_check:
+        w = small_check
+        T, F, D, B = w["T"], w["F"], w["D"], w["B"]
+        times = []
+        for _ in range(101):
+            t0 = time.perf_counter()
+            J = bo.form_beams(w["E"], w["A"], w["G"], T, F, D, B, nthreads=threads)
+            times.append(time.perf_counter() - t0)
+        med = float(np.median(times))
+        out["small"] = {"bit_exact_vs_oracle": bool(np.array_equal(J, w["J"])),
+                        "oracle_median_us": round(med * 1e6, 1), "runs": len(times),
+                        "gpu_us_per_launch": round(w["us_per_launch"], 2),
+                        "oracle_tops": round(workload_ops(T, F, D, B) / med / 1e12, 5),
+                        "threads": min(threads, B)}
+    return out
+
+
+def host_cpu():
+    """CPU model name (from /proc/cpuinfo), logical CPUs of the host and the
+    CPUs this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cpus_allowed": allowed}
 
 
 def main():
@@ -626,14 +730,15 @@ def main():
         log(f"power: {power}")
 
     batched = serving = ingest = others = host_path = None
+    batched_check = serving_check = small_check = None
     if not args.full_only:
         host_path = run_host_path(bf, torch, plan, dE, shape)
         log(f"host path: {host_path}")
-        others = run_other_configs(bf, torch, stream, args.steps, args.warmup)
+        others, small_check = run_other_configs(bf, torch, stream, args.steps, args.warmup)
         log(f"other configurations: {others}")
-        batched = run_batched(bf, torch, stream, max(5, args.steps // 5), 3)
+        batched, batched_check = run_batched(bf, torch, stream, max(5, args.steps // 5), 3)
         log(f"batched: {batched}")
-        serving = run_serving(bf, torch, stream, seconds=args.serve_seconds)
+        serving, serving_check = run_serving(bf, torch, stream, seconds=args.serve_seconds)
         log(f"serving: {serving}")
         if F % 4 == 0 and T >= 8192:
             ingest = run_ingest(bf, torch, stream, dE, A, G, T, F, D, B)
@@ -647,6 +752,18 @@ def main():
         log(f"cpu: {cpu['value']:.4f} TOPS in {cpu['seconds']:.2f} s, parity={cpu['parity_vs_gpu']}")
         cpu["reference_cpu_cxx"] = cpu_reference()
         log(f"cpu reference: {cpu['reference_cpu_cxx']}")
+        cpu.update(host_cpu())
+        checks = oracle_checks(batched_check, serving_check, small_check, cpu["threads_requested"])
+        log(f"oracle checks: {checks}")
+        if batched:
+            batched["bit_exact_vs_oracle"] = checks["batched"]["bit_exact_vs_oracle"]
+            batched["oracle_check"] = checks["batched"]
+        if serving:
+            serving["bit_exact_vs_oracle"] = checks["serving"]["bit_exact_vs_oracle"]
+            serving["oracle_check"] = checks["serving"]
+        if others:
+            others["small"]["cpu_oracle"] = checks["small"]
+        batched_check = serving_check = small_check = None
 
     traffic = load_traffic(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                                         "form_beams_full_traffic.json"))

@@ -25,7 +25,10 @@
  *   J[b][f][t][p]  (J2linear, arraysizes.hxx:103-112; BF_J_BFTP)
  *
  * Streams are hipStream_t values passed as void*; NULL is the null stream.
- * All device pointers are plain hipMalloc'd (or torch) device memory.
+ * All device pointers are plain hipMalloc'd (or torch) device memory. E and J
+ * may start at any byte address: the kernels use unaligned vector accesses.
+ * Every E and J buffer must hold the bytes its shape gives (a chunk or a
+ * request: exactly its own ntimes samples); the library cannot check sizes.
  */
 #ifndef BEAMFORM_H
 #define BEAMFORM_H
@@ -103,14 +106,20 @@ int bf_plan_describe(const bf_plan *plan, char *buf, size_t len);
 /* form_beams on device buffers, asynchronous and ordered on `stream`:
  * J (B*F*P*T bytes) <- E (T*F*D*P bytes). The GPU form of cpu.cxx:15-46 /
  * cuda.cu:35-253; J is fully overwritten. A plan is immutable while in use,
- * so calls on different streams may run concurrently (distinct J buffers). */
+ * so calls on different streams may run concurrently (distinct J buffers).
+ * Shapes with P = 2, D a multiple of 8 up to 1024 and gains G with
+ * |G| * 128 D < 2^30 run the int8 MFMA kernel at any alignment of E and J;
+ * other shapes and gains run a one-thread-per-output kernel with the same
+ * results (bf_plan_describe names the kernel of a plan). */
 int bf_form_beams_device(const bf_plan *plan, uint8_t *J, const uint8_t *E, void *stream);
 
 /* form_beams on host buffers: copies E in, runs, copies J out, synchronously
  * (the flow of cuda.cu:274-314). The time axis goes through the device in
  * slabs, the copies in, the kernels and the copies out on three streams, so
  * that they overlap (best with pinned host buffers); device memory use is
- * two slabs of about T / 8 samples each, not whole E and J. */
+ * two slabs of about T / 8 samples each, not whole E and J. The streams,
+ * events and slab buffers are made on a plan's first call and kept with the
+ * plan; calls on one plan from several threads run one after the other. */
 int bf_form_beams_host(const bf_plan *plan, uint8_t *J, const uint8_t *E);
 
 /* One-shot drop-in for the reference entry point with an explicit shape, all
@@ -118,7 +127,10 @@ int bf_form_beams_host(const bf_plan *plan, uint8_t *J, const uint8_t *E);
 int bf_form_beams(uint8_t *J, const uint8_t *E, const uint8_t *A, const float *G, const bf_shape *shape);
 
 /* Adler-32 (adler32.c:7-20) of `len` bytes of DEVICE memory, computed on the
- * device; the result is written to *out (host) after the stream has run it. */
+ * device; the result is written to *out (host) after the stream has run it.
+ * Its 32 KiB of device scratch comes from a per-device pool kept by the
+ * library (allocated on first use, reused by later calls; concurrent calls
+ * get distinct scratch). */
 int bf_adler32_device(const uint8_t *data, size_t len, uint32_t *out, void *stream);
 
 /* Adler-32 of host memory (adler32.c:7-20), for the checksum flow of
@@ -154,8 +166,9 @@ int bf_form_beams_channels(const bf_plan *plan, int channel, int nfreq, int64_t 
                            const uint8_t *E, void *stream);
 
 typedef struct bf_chunk {
-  const uint8_t *E; /* device: [ntimes][nfreq][D][P] (Elinear with F = nfreq) */
-  uint8_t *J;       /* device: [B][nfreq][P][ntimes] or [B][nfreq][ntimes][P] per the plan layout */
+  const uint8_t *E; /* device: [ntimes][nfreq][D][P] (Elinear with F = nfreq), exactly ntimes samples */
+  uint8_t *J;       /* device: [B][nfreq][P][ntimes] or [B][nfreq][ntimes][P] per the plan layout,
+                       exactly ntimes samples (the kernel addresses rows with this chunk's ntimes) */
   int64_t ntimes;   /* >= 1 */
   int32_t channel;  /* first row of the plan's channel table used by the chunk */
   int32_t nfreq;    /* consecutive channels channel .. channel + nfreq - 1 (0 means 1); <= 65535 */
@@ -259,7 +272,7 @@ typedef struct bf_server_config {
 } bf_server_config;
 
 typedef struct bf_request {
-  const uint8_t *E;  /* [ntimes][nfreq][D][P]: device memory (16-byte aligned), or host if e_on_host */
+  const uint8_t *E;  /* [ntimes][nfreq][D][P]: device memory, or host if e_on_host */
   int64_t ntimes;    /* >= 1 */
   int32_t channel;   /* first channel-table row */
   int32_t nfreq;     /* consecutive channels (0 means 1) */

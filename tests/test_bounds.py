@@ -1,0 +1,29 @@
+"""Host check of the gfx950 kernels' memory accesses (tests/bounds_check.cpp):
+the tile pipeline's address arithmetic, shared with the kernels through
+bf_device.hpp, run for every wave and lane of plain and batched launches over
+random shapes and chunk lists. Every E load and J store must stay inside its
+buffer and every J byte must be written exactly once. Compiled for the CPU
+with hipcc (no GPU)."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.join(HERE, "..")
+CSRC = os.path.join(ROOT, "beamforming_amd", "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+def test_kernel_accesses_stay_in_bounds(tmp_path):
+    exe = tmp_path / "bounds_check"
+    cmd = [HIPCC, "-std=c++17", "-O2", "-Wall", "-I", CSRC, "-I", INCLUDE, os.path.join(HERE, "bounds_check.cpp"),
+           "-o", str(exe)]
+    build = subprocess.run(cmd, capture_output=True, text=True)
+    assert build.returncode == 0, build.stderr
+    run = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert run.returncode == 0, run.stdout + run.stderr
+    assert run.stdout.strip().endswith("ok")

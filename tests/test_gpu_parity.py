@@ -268,6 +268,39 @@ def test_host_entry_points():
         assert np.array_equal(plan.form_beams_host(E), want)
 
 
+@pytest.mark.parametrize("e_off,j_off", [(1, 0), (0, 3), (7, 5), (13, 1)])
+@pytest.mark.parametrize("layout", [bf.BF_J_BFPT, bf.BF_J_BFTP])
+def test_unaligned_buffers_take_the_mfma_kernel(e_off, j_off, layout):
+    # E and J at odd byte addresses: the MFMA kernel (unaligned vector loads
+    # and stores), bit-exact, with the bytes around J untouched
+    T, F, D, B = 777, 3, 512, 96
+    shape = bf.Shape.make(T, F, D, B, j_layout=layout)
+    E, A, G = seeded(T, F, D, B, 71 + e_off)
+    with bf.Plan(shape, A, G) as plan:
+        assert plan.describe().startswith("mfma")
+        dE = torch.zeros(shape.e_bytes + 32, dtype=torch.uint8, device="cuda")
+        dE[e_off:e_off + shape.e_bytes] = torch.from_numpy(E).cuda()
+        dJ = torch.zeros(shape.j_bytes + 32, dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        plan.form_beams_device(dJ.data_ptr() + j_off, dE.data_ptr() + e_off, stream())
+        torch.cuda.synchronize()
+        J = dJ.cpu().numpy()
+    assert (J[:j_off] == 0).all() and (J[j_off + shape.j_bytes:] == 0).all(), "bytes outside J were written"
+    assert np.array_equal(J[j_off:j_off + shape.j_bytes], bo.form_beams(E, A, G, T, F, D, B, jlayout=layout))
+
+
+def test_host_path_reused_across_calls():
+    # the plan keeps the host path's streams, events and slab buffers: calls
+    # after the first reuse them, with new inputs each time
+    T, F, D, B = 3000, 2, 256, 64
+    shape = bf.Shape.make(T, F, D, B)
+    _, A, G = seeded(T, F, D, B, 81)
+    with bf.Plan(shape, A, G) as plan:
+        for k in range(3):
+            E = bo.fill_uniform(T * F * D * 2, 82 + k)
+            assert np.array_equal(plan.form_beams_host(E), bo.form_beams(E, A, G, T, F, D, B)), f"call {k}"
+
+
 @pytest.mark.parametrize("T,F,D,B,layout,pinned", [
     (1000, 3, 64, 40, bf.BF_J_BFPT, False),    # 4 slabs of 256 samples, the last 232
     (4099, 2, 128, 96, bf.BF_J_BFTP, True),    # 8 slabs of 528, the last 403; pinned buffers

@@ -167,8 +167,6 @@ def test_bad_requests_and_shutdown(table):
         with pytest.raises(bf.BeamformError) as e:
             srv.submit(q.dE.data_ptr(), bad["ntimes"], bad["channel"], bad["nfreq"])
         assert e.value.status == bf.BF_ERR_INVALID_ARGUMENT
-    with pytest.raises(bf.BeamformError):
-        srv.submit(q.dE.data_ptr() + 4, 64, 0, 1)  # misaligned device E
     # shut down with work queued and results unreleased: must return promptly
     big = Req(16384, 0, 4, 10)
     torch.cuda.synchronize()
@@ -250,6 +248,55 @@ def test_submitters_on_several_threads(table):
         for i, q in enumerate(reqs):
             want = oracle_request(q.E, A, G, q.T, q.channel, q.nf)
             assert np.array_equal(got[100 * k + i][1], want), f"thread {k} request {i}"
+
+
+def test_submitters_race_the_completion_scan(table, monkeypatch):
+    # A submitting thread (poll_us 0) takes a launch slot, drops the server
+    # lock and only then enqueues the launch and records its completion
+    # event; the loop thread's completion scan must leave such a slot alone
+    # (before the event is recorded it reads as complete). The library's
+    # BF_SERVER_ISSUE_DELAY_US pause widens that window to 3 ms, and four
+    # threads submit at once: a launch finished early would hand out J before
+    # the kernel wrote it, which the oracle comparison catches.
+    import threading
+    monkeypatch.setenv("BF_SERVER_ISSUE_DELAY_US", "3000")
+    A, G, plan = table
+    groups = [random_reqs(6, 120 + k, tmax=3000) for k in range(4)]
+    torch.cuda.synchronize()
+    errors = []
+    with bf.Server(plan, hbm_budget=1 << 29, poll_us=0) as srv:
+        def submit_all(k):
+            try:
+                for i, q in enumerate(groups[k]):
+                    srv.submit(q.dE.data_ptr(), q.T, q.channel, q.nf, tag=100 * k + i)
+            except Exception as exc:  # reported below
+                errors.append(exc)
+        threads = [threading.Thread(target=submit_all, args=(k,)) for k in range(4)]
+        for t in threads:
+            t.start()
+        got = collect(srv, 24)
+        for t in threads:
+            t.join()
+        st = srv.stats()
+    say(f"stats {st}")
+    assert not errors, errors
+    assert st["completed"] == 24 and st["failed"] == 0
+    for k, reqs in enumerate(groups):
+        for i, q in enumerate(reqs):
+            want = oracle_request(q.E, A, G, q.T, q.channel, q.nf)
+            assert np.array_equal(got[100 * k + i][1], want), f"thread {k} request {i}"
+
+
+def test_unaligned_requests(table):
+    A, G, plan = table
+    q = Req(2500, 3, 4, 88)
+    dE = torch.zeros(q.E.size + 32, dtype=torch.uint8, device="cuda")
+    dE[5:5 + q.E.size] = q.dE
+    torch.cuda.synchronize()
+    with bf.Server(plan, hbm_budget=1 << 28) as srv:
+        srv.submit(dE.data_ptr() + 5, q.T, q.channel, q.nf, tag=1)
+        got = collect(srv, 1)
+    assert np.array_equal(got[1][1], oracle_request(q.E, A, G, q.T, q.channel, q.nf))
 
 
 def test_j2_layout_plan_through_the_server():
