@@ -18,7 +18,7 @@ import numpy as np
 
 __all__ = [
     "BF_J_BFPT", "BF_J_BFTP", "BeamformError", "Shape", "Plan", "lib", "library_path",
-    "form_beams", "adler32", "adler32_device", "fill_uniform_device", "version", "Chunk", "Batch",
+    "form_beams", "adler32", "adler32_device", "adler32_device_noalloc", "fill_uniform_device", "version", "Chunk", "Batch",
     "Server", "ServerConfig", "Request", "Result", "ServerCounters", "Ingest", "IngestConfig",
     "BF_ERR_INVALID_ARGUMENT", "BF_ERR_UNSUPPORTED", "BF_ERR_OVER_BUDGET", "BF_ERR_NOT_READY", "BF_ERR_CLOSED",
 ]
@@ -193,6 +193,7 @@ def _load():
         "bf_form_beams_host": (ctypes.c_int, [vp, u8p, u8p]),
         "bf_form_beams": (ctypes.c_int, [u8p, u8p, u8p, fp, ctypes.POINTER(Shape)]),
         "bf_adler32_device": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32), vp]),
+        "bf_adler32_device_noalloc": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32), vp]),
         "bf_adler32": (ctypes.c_uint32, [u8p, ctypes.c_size_t]),
         "bf_fill_uniform_device": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64, vp]),
         "bf_form_beams_channel": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int64, u8p, u8p, vp]),
@@ -362,6 +363,14 @@ def adler32_device(ptr, nbytes, stream=0):
     """Adler-32 of device bytes, computed on the device."""
     out = ctypes.c_uint32()
     _check(lib.bf_adler32_device(ctypes.c_void_p(ptr), nbytes, ctypes.byref(out), ctypes.c_void_p(stream)))
+    return int(out.value)
+
+
+def adler32_device_noalloc(ptr, nbytes, stream=0):
+    """Adler-32 of device bytes, computed on the device with the library's
+    static scratch (no allocation; calls overlapping on one GPU race)."""
+    out = ctypes.c_uint32()
+    _check(lib.bf_adler32_device_noalloc(ctypes.c_void_p(ptr), nbytes, ctypes.byref(out), ctypes.c_void_p(stream)))
     return int(out.value)
 
 

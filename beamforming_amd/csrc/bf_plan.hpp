@@ -92,5 +92,6 @@ int prepare_weights(bf_plan *p, hipStream_t st);
 int launch_mfma(const bf_plan *p, const MfmaConfig &c, uint8_t *J, const uint8_t *E, int64_t T, int F, int wrow0,
                 hipStream_t st);
 int launch_form_beams(const bf_plan *p, uint8_t *J, const uint8_t *E, hipStream_t st, bool allow_narrow = true);
+int adler32_run(const uint8_t *data, size_t len, unsigned long long *scratch, uint32_t *out, hipStream_t st);
 bool pick_config(const bf_plan *p, int64_t T, int64_t F, MfmaConfig *c);
 } // namespace bf

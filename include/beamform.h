@@ -133,6 +133,15 @@ int bf_form_beams(uint8_t *J, const uint8_t *E, const uint8_t *A, const float *G
  * get distinct scratch). */
 int bf_adler32_device(const uint8_t *data, size_t len, uint32_t *out, void *stream);
 
+/* Same as bf_adler32_device, allocating nothing: no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). Its
+ * 32 KiB + 16 bytes of scratch is a static device array of the library, one
+ * per GPU, held until the process ends. Any len is accepted (the scratch
+ * does not grow with it). Calls that overlap on one GPU share that scratch
+ * and race; calls one after the other on one stream are safe. */
+int bf_adler32_device_noalloc(const uint8_t *data, size_t len, uint32_t *out, void *stream);
+
 /* Adler-32 of host memory (adler32.c:7-20), for the checksum flow of
  * arraysizes.cxx:62-88 without a device round trip. */
 uint32_t bf_adler32(const uint8_t *data, size_t len);
