@@ -223,6 +223,8 @@ def _load():
         "bf_ingest_create": (ctypes.c_int, [ctypes.POINTER(IngestConfig), ctypes.POINTER(vp)]),
         "bf_ingest_destroy": (ctypes.c_int, [vp]),
         "bf_ingest_push": (ctypes.c_int, [vp, u8p, ctypes.c_int64, ctypes.c_int]),
+        "bf_ingest_push_many": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_int64),
+                                               ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
         "bf_ingest_pop": (ctypes.c_int, [vp, vp, ctypes.POINTER(Chunk), ctypes.POINTER(ctypes.c_uint64)]),
         "bf_ingest_release": (ctypes.c_int, [vp, ctypes.c_uint64, vp]),
         "bf_ingest_pending": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)]),
@@ -561,6 +563,18 @@ class Ingest:
             return False
         _check(st)
         return True
+
+    def push_many(self, packet_ptrs, ntimes, on_host):
+        """Pushes packets in order until one is refused; returns how many were
+        taken (fewer than given when a ring slot is missing)."""
+        n = len(packet_ptrs)
+        ptrs = (ctypes.c_void_p * max(1, n))(*packet_ptrs)
+        nts = (ctypes.c_int64 * max(1, n))(*ntimes)
+        pushed = ctypes.c_int()
+        st = lib.bf_ingest_push_many(self._h, ptrs, nts, n, int(bool(on_host)), ctypes.byref(pushed))
+        if st not in (0, BF_ERR_NOT_READY):
+            _check(st)
+        return pushed.value
 
     def pop(self, stream=0):
         """Next complete chunk as (Chunk, slot_id), or None."""

@@ -5,16 +5,19 @@
 // the batch and serving paths take (bf_chunk with nfreq = G).
 //
 // The corner turn runs on the ingest's own stream. Device-resident packets
-// (16-byte aligned) go through one scatter kernel per packet segment: every
-// thread moves 16-byte pieces from a packet row to the row of its group's
-// slot, reading and writing whole contiguous rows, so a packet costs one
-// launch whatever the number of groups. Host packets (and unaligned device
-// packets) go through the copy engines: for each group, one strided 2-D copy
-// per packet segment (source pitch F D P, destination pitch G D P, width
-// G D P bytes). A chunk is complete when the packets have filled T_chunk
-// samples; bf_ingest_pop hands it out after making the consumer's stream wait
-// for its copies, and bf_ingest_release returns its slot once the consumer's
-// work on it (ordered on a stream) has finished.
+// (16-byte aligned) are copied by a scatter kernel that moves whole packet
+// rows (all groups of one time sample per workgroup) to their groups' slot
+// rows. A push only records its packet segments; one launch copies every
+// recorded segment when a chunk completes (so its `filled` event follows its
+// copies), when bf_ingest_sync is called, or when kMaxPending segments are
+// waiting, so that a stream of small packets costs one launch per chunk, not
+// one per packet. Host packets (and unaligned device packets) go through the
+// copy engines at once: for each group, one strided 2-D copy per packet
+// segment (source pitch F D P, destination pitch G D P, width G D P bytes). A
+// chunk is complete when the packets have filled T_chunk samples;
+// bf_ingest_pop hands it out after making the consumer's stream wait for its
+// copies, and bf_ingest_release returns its slot once the consumer's work on
+// it (ordered on a stream) has finished.
 #include "beamform.h"
 #include "bf_plan.hpp"
 
@@ -45,22 +48,54 @@ struct SlotPick {
   uint8_t slot[kScatterGroups];
 };
 
-// Copies rows t = 0 .. n - 1 of a packet segment: group g's width bytes at
-// packet + t spitch + (grp0 + g) width go to its slot at dst_off + t width.
-// width and spitch are multiples of 16, the packet is 16-byte aligned.
-// blockIdx.x covers the 16-byte pieces of a row's groups (row_units =
-// ngroups width / 16 < 2^31), blockIdx.y strides over the rows.
-__global__ void ingest_scatter_kernel(const uint8_t *__restrict__ packet, int64_t n, int64_t spitch, int64_t width,
-                                      int grp0, int ngroups, uint8_t *const *__restrict__ slot_table, int ring_slots,
-                                      SlotPick pick, int64_t dst_off) {
+// Packet segments copied by one launch: segment k holds n[k] rows starting
+// at src[k] (pitch spitch) that go to rows dst_row[k] .. of the groups' fill
+// slots; rows_before[k] = n[0] + .. + n[k - 1]. All segments of a launch fill
+// the same slots (a launch never spans a chunk boundary).
+constexpr int kMaxPending = 32;
+struct ScatterBatch {
+  const uint8_t *src[kMaxPending];
+  int64_t dst_row[kMaxPending];
+  int64_t rows_before[kMaxPending + 1];
+  int32_t nseg;
+};
+
+// Workgroup r copies row r of the launch (segment k, row r - rows_before[k]):
+// its ngroups * width bytes, contiguous in the packet, go to width bytes in
+// each group's slot. width and spitch are multiples of 16 and the packets are
+// 16-byte aligned; each thread moves kPieces 16-byte pieces per pass, all
+// loads issued before the stores.
+constexpr int kScatterThreads = 256;
+constexpr int kPieces = 4;
+__global__ __launch_bounds__(kScatterThreads) void ingest_scatter_kernel(const ScatterBatch b, int64_t spitch,
+                                                                          int64_t width, int grp0, int ngroups,
+                                                                          uint8_t *const *__restrict__ slot_table,
+                                                                          int ring_slots, SlotPick pick) {
+  const int64_t r = blockIdx.x;
+  int k = 0;
+  while (k + 1 < b.nseg && b.rows_before[k + 1] <= r) ++k; // uniform, at most kMaxPending steps
+  const int64_t t = r - b.rows_before[k];
+  const uint4 *src = reinterpret_cast<const uint4 *>(b.src[k] + t * spitch + (int64_t)grp0 * width);
+  const int64_t dst_off = (b.dst_row[k] + t) * width;
   const uint32_t wu = (uint32_t)(width / 16);
-  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= (uint32_t)ngroups * wu) return;
-  const uint32_t g = u / wu, k = u - g * wu;
-  const uint8_t *src = packet + (int64_t)(grp0 + (int)g) * width + 16 * (int64_t)k;
-  uint8_t *dst = slot_table[(int64_t)(grp0 + (int)g) * ring_slots + pick.slot[g]] + dst_off + 16 * (int64_t)k;
-  for (int64_t t = blockIdx.y; t < n; t += gridDim.y)
-    *reinterpret_cast<uint4 *>(dst + t * width) = *reinterpret_cast<const uint4 *>(src + t * spitch);
+  const uint32_t units = (uint32_t)ngroups * wu;
+  for (uint32_t u0 = threadIdx.x; u0 < units; u0 += kScatterThreads * kPieces) {
+    uint4 v[kPieces];
+#pragma unroll
+    for (int i = 0; i < kPieces; ++i) {
+      const uint32_t u = u0 + (uint32_t)(i * kScatterThreads);
+      if (u < units) v[i] = src[u];
+    }
+#pragma unroll
+    for (int i = 0; i < kPieces; ++i) {
+      const uint32_t u = u0 + (uint32_t)(i * kScatterThreads);
+      if (u < units) {
+        const uint32_t g = u / wu, q = u - g * wu;
+        uint8_t *dst = slot_table[(int64_t)(grp0 + (int)g) * ring_slots + pick.slot[g]] + dst_off + 16 * (int64_t)q;
+        *reinterpret_cast<uint4 *>(dst) = v[i];
+      }
+    }
+  }
 }
 
 } // namespace
@@ -76,6 +111,12 @@ struct bf_ingest {
   std::vector<int> fill_slot;           // slot being filled, per group (-1: none)
   int64_t fill_pos = 0;                 // samples already in the filling slots (same for every group)
   std::deque<Ready> ready;
+  ScatterBatch pending{};               // device packet segments recorded, not yet copied
+  int64_t pending_rows = 0;
+  // Device allocation of the last device packet ([dev_base, dev_base +
+  // dev_size)): later packets inside it skip the pointer-attribute query.
+  uintptr_t dev_base = 0;
+  size_t dev_size = 0;
 };
 
 namespace {
@@ -95,6 +136,28 @@ bool slot_free(Slot &s) {
     return true;
   }
   return false;
+}
+
+// Launches the copies of the recorded device packet segments (into the
+// groups' current fill slots) and clears the record.
+int flush_pending(bf_ingest *g) {
+  ScatterBatch &b = g->pending;
+  if (b.nseg == 0) return BF_OK;
+  const bf_ingest_config &c = g->cfg;
+  const int64_t spitch = c.nfrequencies * (int64_t)g->row_bytes;
+  const int64_t width = c.group_channels * (int64_t)g->row_bytes;
+  b.rows_before[b.nseg] = g->pending_rows;
+  for (int grp0 = 0; grp0 < g->ngroups; grp0 += kScatterGroups) {
+    const int ng = std::min(kScatterGroups, g->ngroups - grp0);
+    SlotPick pick;
+    for (int i = 0; i < ng; ++i) pick.slot[i] = (uint8_t)g->fill_slot[grp0 + i];
+    hipLaunchKernelGGL(ingest_scatter_kernel, dim3((unsigned)g->pending_rows), dim3(kScatterThreads), 0, g->stream, b,
+                       spitch, width, grp0, ng, g->d_slot_table, c.ring_slots, pick);
+    BF_HIP(hipGetLastError());
+  }
+  b.nseg = 0;
+  g->pending_rows = 0;
+  return BF_OK;
 }
 
 } // namespace
@@ -202,10 +265,24 @@ int bf_ingest_push(bf_ingest *g, const uint8_t *packet, int64_t ntimes, int pack
   // 16 because D is a multiple of 8)
   bool scatter = !packet_on_host && ((uintptr_t)packet & 15u) == 0 && c.ring_slots <= 255;
   if (scatter) { // only memory the kernel can read for sure: device allocations
-    hipPointerAttribute_t attr;
-    scatter = hipPointerGetAttributes(&attr, packet) == hipSuccess &&
-              (attr.type == hipMemoryTypeDevice || attr.isManaged);
-    (void)hipGetLastError(); // a failed query of a host pointer is not an error here
+    const uintptr_t p = (uintptr_t)packet;
+    const size_t bytes = (size_t)ntimes * (size_t)c.nfrequencies * g->row_bytes;
+    if (!(g->dev_size && p >= g->dev_base && p + bytes <= g->dev_base + g->dev_size)) {
+      hipPointerAttribute_t attr;
+      scatter = hipPointerGetAttributes(&attr, packet) == hipSuccess &&
+                (attr.type == hipMemoryTypeDevice || attr.isManaged);
+      (void)hipGetLastError(); // a failed query of a host pointer is not an error here
+      void *base = nullptr;
+      size_t size = 0;
+      if (scatter && hipMemGetAddressRange(&base, &size, const_cast<uint8_t *>(packet)) == hipSuccess &&
+          p + bytes <= (uintptr_t)base + size) {
+        g->dev_base = (uintptr_t)base;
+        g->dev_size = size;
+      } else {
+        (void)hipGetLastError();
+        g->dev_size = 0;
+      }
+    }
   }
   const size_t spitch = (size_t)c.nfrequencies * g->row_bytes;
   const size_t width = (size_t)c.group_channels * g->row_bytes;
@@ -221,17 +298,15 @@ int bf_ingest_push(bf_ingest *g, const uint8_t *packet, int64_t ntimes, int pack
       }
     const int64_t n = std::min(ntimes - done, c.chunk_ntimes - g->fill_pos);
     if (scatter) {
-      for (int grp0 = 0; grp0 < g->ngroups; grp0 += kScatterGroups) {
-        const int ng = std::min(kScatterGroups, g->ngroups - grp0);
-        SlotPick pick;
-        for (int i = 0; i < ng; ++i) pick.slot[i] = (uint8_t)g->fill_slot[grp0 + i];
-        const int64_t row_units = (int64_t)ng * (int64_t)(width / 16); // < 2^31, checked at creation
-        const dim3 grid((unsigned)((row_units + 255) / 256), (unsigned)std::min<int64_t>(n, 65535));
-        hipLaunchKernelGGL(ingest_scatter_kernel, grid, dim3(256), 0, g->stream, packet + (size_t)done * spitch,
-                           n, (int64_t)spitch, (int64_t)width, grp0, ng, g->d_slot_table, c.ring_slots, pick,
-                           (int64_t)g->fill_pos * (int64_t)width);
-        BF_HIP(hipGetLastError());
-      }
+      // recorded; copied with the other segments of this chunk in one launch
+      if (g->pending.nseg == kMaxPending || g->pending_rows + n > (int64_t)0x7fffffff)
+        if (int rc = flush_pending(g)) return rc;
+      ScatterBatch &b = g->pending;
+      b.src[b.nseg] = packet + (size_t)done * spitch;
+      b.dst_row[b.nseg] = g->fill_pos;
+      b.rows_before[b.nseg] = g->pending_rows;
+      ++b.nseg;
+      g->pending_rows += n;
     } else {
       for (int grp = 0; grp < g->ngroups; ++grp) {
         Slot &s = g->slots[grp][g->fill_slot[grp]];
@@ -243,6 +318,7 @@ int bf_ingest_push(bf_ingest *g, const uint8_t *packet, int64_t ntimes, int pack
     done += n;
     g->fill_pos += n;
     if (g->fill_pos == c.chunk_ntimes) {
+      if (int rc = flush_pending(g)) return rc; // the chunk's copies, before its `filled` events
       for (int grp = 0; grp < g->ngroups; ++grp) {
         BF_HIP(hipEventRecord(g->slots[grp][g->fill_slot[grp]].filled, g->stream));
         g->ready.push_back(Ready{grp, g->fill_slot[grp]});
@@ -250,6 +326,18 @@ int bf_ingest_push(bf_ingest *g, const uint8_t *packet, int64_t ntimes, int pack
       }
       g->fill_pos = 0;
     }
+  }
+  return BF_OK;
+}
+
+int bf_ingest_push_many(bf_ingest *g, const uint8_t *const *packets, const int64_t *ntimes, int n,
+                        int packet_on_host, int *pushed) {
+  if (pushed) *pushed = 0;
+  if (!g || (n > 0 && (!packets || !ntimes)) || n < 0) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument or n < 0");
+  for (int i = 0; i < n; ++i) {
+    const int rc = bf_ingest_push(g, packets[i], ntimes[i], packet_on_host);
+    if (rc) return rc;
+    if (pushed) *pushed = i + 1;
   }
   return BF_OK;
 }
@@ -286,6 +374,7 @@ int bf_ingest_release(bf_ingest *g, uint64_t slot_id, void *stream) {
 
 int bf_ingest_sync(bf_ingest *g) {
   if (!g) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (int rc = flush_pending(g)) return rc;
   BF_HIP(hipStreamSynchronize(g->stream));
   return BF_OK;
 }

@@ -311,6 +311,12 @@ def run_batched(bf, torch, stream, steps, warmup, seed=7, C=64, D=512, B=96):
     # every chunk's J (the alignment gaps between chunks are never written)
     same = all(bool(torch.equal(dJb[j_off[c]:j_off[c] + B * 2 * Ts[c]], dJp[j_off[c]:j_off[c] + B * 2 * Ts[c]]))
                for c in range(C))
+    # the same rows as one plain launch of channel 0 over sum(T) samples (the
+    # batch's E buffer and the plain launches' J buffer reused; timing only):
+    # the cost of the work without the batch's uneven chunks
+    sumT_all = sum(Ts)
+    ms_one = timed(stream, steps, warmup,
+                   lambda: plan.form_beams_channel(0, sumT_all, dJp.data_ptr(), dE.data_ptr(), sptr), settle_s=0.2)
     sumT = sum(Ts)
     ops = workload_ops(sumT, 1, D, B)
     best = min(ms_batch, ms_graph)
@@ -325,6 +331,8 @@ def run_batched(bf, torch, stream, steps, warmup, seed=7, C=64, D=512, B=96):
         "ms_graph": round(ms_graph, 4),
         "ms_plain_per_chunk": round(ms_plain, 4),
         "speedup_vs_plain": round(ms_plain / best, 3),
+        "ms_one_plain_launch_over_sum_ntimes": round(ms_one, 4),
+        "batch_overhead_vs_one_launch": round(best / ms_one - 1.0, 4),
         "bit_exact_vs_plain": same,
         "batch": batch.describe(),
         "autotune": tune.strip().splitlines()[-1],
@@ -357,7 +365,7 @@ def run_host_path(bf, torch, plan, dE, shape, reps=3):
     }
 
 
-def run_other_configs(bf, torch, stream, steps, warmup):
+def run_other_configs(bf, torch, stream, steps, warmup, names=("small", "wide", "full_j2_layout")):
     """The other single-GPU configurations, each timed like the full-size
     launch: small (T=256 F=4 D=64 B=16, seeded uniform E and A, bound by the
     launch latency), wide (T=8192 F=16 D=512 B=512, uniform E, phase-ramp
@@ -365,9 +373,12 @@ def run_other_configs(bf, torch, stream, steps, warmup):
     J[b][f][t][p]."""
     sptr = stream.cuda_stream
     out = {}
+    small_check = None
     for name, (T, F, D, B), ramp, layout in [("small", (256, 4, 64, 16), False, 0),
                                              ("wide", (8192, 16, 512, 512), True, 0),
                                              ("full_j2_layout", (32768, 32, 512, 96), False, 1)]:
+        if name not in names:
+            continue
         shape = bf.Shape.make(T, F, D, B, j_layout=layout)
         A = phase_ramp_weights(F, B, D) if ramp else host_uniform(bf, torch, F * B * D, 21)
         G = reference_gains(F * B, D)
@@ -412,17 +423,22 @@ def run_ingest(bf, torch, stream, dE, A, G, T, F, D, B, packet=256, TC=8192, gro
     Js = [torch.empty(B * groups * 2 * TC, dtype=torch.uint8, device="cuda") for _ in range(ng)]
 
     def pump(ing, src, on_host, t_total, beams):
-        t, popped, stalls = 0, [], 0
-        while t < t_total:
-            n = min(packet, t_total - t)
-            if not ing.push(src + t * row, n, on_host):
+        # the packets of one chunk time go in per call (bf_ingest_push_many),
+        # then the completed chunks are popped
+        starts = list(range(0, t_total, packet))
+        per_call = max(1, TC // packet)
+        i, popped, stalls = 0, [], 0
+        while i < len(starts):
+            group = starts[i:i + per_call]
+            k = ing.push_many([src + t * row for t in group], [min(packet, t_total - t) for t in group], on_host)
+            if k == 0:
                 stalls += 1
                 if stalls > 2:
                     raise RuntimeError("ingest: push refused with every slot released")
                 torch.cuda.synchronize()  # the consumer's releases complete
                 continue
             stalls = 0
-            t += n
+            i += k
             while True:
                 r = ing.pop(sptr)
                 if r is None:
@@ -464,8 +480,9 @@ def run_ingest(bf, torch, stream, dE, A, G, T, F, D, B, packet=256, TC=8192, gro
         "with_beams_beam_samples_per_s": B * F * nt * 2 / dt_b,
         "host_fed_ingest_only_gbps": round(TC * row / dt_c / 1e9, 1),
         "host_fed_sample": f"{TC} samples ({TC * row / 2**20:.0f} MiB) from pinned host memory",
-        "note": "wall clock from packet push to the last chunk's release (device synchronised on both sides); "
-                "hbm_frac counts the copy's read and write",
+        "note": "wall clock from packet push to the last chunk's release (device synchronised on both sides), "
+                "the packets of one chunk time pushed per bf_ingest_push_many call; hbm_frac counts the copy's "
+                "read and write",
     }
 
 
@@ -661,7 +678,11 @@ def main():
     ap.add_argument("--nbeams", type=int, default=96)
     ap.add_argument("--cpu-frequencies", type=int, default=32, help="frequencies in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--full-only", action="store_true", help="skip the batched and serving workloads")
+    ap.add_argument("--full-only", action="store_true", help="skip every workload but the full-size launch")
+    ap.add_argument("--legs", default="host,others,batched,serving,ingest",
+                    help="comma-separated workloads run after the full-size launch (for profiling one of them)")
+    ap.add_argument("--configs", default="small,wide,full_j2_layout",
+                    help="comma-separated configurations of the 'others' leg")
     ap.add_argument("--autotune", action="store_true",
                     help="autotune the full-size kernel configuration before timing (default: the built-in choice)")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
@@ -731,15 +752,21 @@ def main():
 
     batched = serving = ingest = others = host_path = None
     batched_check = serving_check = small_check = None
-    if not args.full_only:
+    legs = set() if args.full_only else {x.strip() for x in args.legs.split(",") if x.strip()}
+    if "host" in legs:
         host_path = run_host_path(bf, torch, plan, dE, shape)
         log(f"host path: {host_path}")
-        others, small_check = run_other_configs(bf, torch, stream, args.steps, args.warmup)
+    if "others" in legs:
+        others, small_check = run_other_configs(bf, torch, stream, args.steps, args.warmup,
+                                                names=[x.strip() for x in args.configs.split(",")])
         log(f"other configurations: {others}")
+    if "batched" in legs:
         batched, batched_check = run_batched(bf, torch, stream, max(5, args.steps // 5), 3)
         log(f"batched: {batched}")
+    if "serving" in legs:
         serving, serving_check = run_serving(bf, torch, stream, seconds=args.serve_seconds)
         log(f"serving: {serving}")
+    if "ingest" in legs:
         if F % 4 == 0 and T >= 8192:
             ingest = run_ingest(bf, torch, stream, dE, A, G, T, F, D, B)
             ingest["plain_launch_beam_samples_per_s"] = B * F * T * 2 / (ms * 1e-3)
@@ -755,13 +782,13 @@ def main():
         cpu.update(host_cpu())
         checks = oracle_checks(batched_check, serving_check, small_check, cpu["threads_requested"])
         log(f"oracle checks: {checks}")
-        if batched:
+        if batched and "batched" in checks:
             batched["bit_exact_vs_oracle"] = checks["batched"]["bit_exact_vs_oracle"]
             batched["oracle_check"] = checks["batched"]
-        if serving:
+        if serving and "serving" in checks:
             serving["bit_exact_vs_oracle"] = checks["serving"]["bit_exact_vs_oracle"]
             serving["oracle_check"] = checks["serving"]
-        if others:
+        if others and "small" in checks:
             others["small"]["cpu_oracle"] = checks["small"]
         batched_check = serving_check = small_check = None
 

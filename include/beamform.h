@@ -392,6 +392,13 @@ int bf_ingest_destroy(bf_ingest *ingest);
  * that point). */
 int bf_ingest_push(bf_ingest *ingest, const uint8_t *packet, int64_t ntimes, int packet_on_host);
 
+/* Pushes packets[0 .. n-1] (ntimes[i] samples each) in order, as n calls of
+ * bf_ingest_push, stopping at the first one refused: its status is returned
+ * (BF_ERR_NOT_READY when a ring slot is missing) and *pushed (may be NULL)
+ * counts the packets taken. For receivers that get several packets at once. */
+int bf_ingest_push_many(bf_ingest *ingest, const uint8_t *const *packets, const int64_t *ntimes, int n,
+                        int packet_on_host, int *pushed);
+
 /* Waits until every packet pushed so far has been read; the caller may then
  * reuse the packet buffers. */
 int bf_ingest_sync(bf_ingest *ingest);

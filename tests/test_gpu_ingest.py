@@ -127,3 +127,31 @@ def test_ingest_many_groups():
         k = i // F
         assert nt == TC and ch == i % F
         assert np.array_equal(data, E[k * TC:(k + 1) * TC, ch].reshape(-1)), f"chunk {i}"
+
+
+def test_push_many_and_partial_refusal():
+    # packets pushed several per call: the chunks are the packets' slices; with
+    # two ring slots and nothing released, the call stops at the first packet
+    # that needs a third slot and reports how many it took
+    F, D, G, TC = 8, 64, 2, 64
+    total = 3 * TC
+    E = bo.fill_uniform(total * F * D * 2, 74).reshape(total, F, D * 2)
+    dE = torch.from_numpy(E.reshape(-1)).cuda()
+    row = F * D * 2
+    starts = list(range(0, total, 16))
+    with bf.Ingest(F, D, TC, G, ring_slots=2) as ing:
+        k = ing.push_many([dE.data_ptr() + t * row for t in starts], [16] * len(starts), False)
+        assert k == 2 * TC // 16  # the third chunk has no free slot
+        ing.sync()
+        got = []
+        while True:
+            r = ing.pop(stream())
+            if r is None:
+                break
+            c, sid = r
+            torch.cuda.synchronize()
+            got.append((c.channel, device_bytes(c.E, TC * G * D * 2)))
+        assert len(got) == 2 * (F // G)
+        for i, (ch, data) in enumerate(got):
+            kk = i // (F // G)
+            assert np.array_equal(data, E[kk * TC:(kk + 1) * TC, ch:ch + G].reshape(-1)), f"chunk {i}"
