@@ -570,7 +570,7 @@ int bf_plan_describe(const bf_plan *p, char *buf, size_t len) {
   if (p->use_mfma) {
     const MfmaConfig &c = p->cfg;
     std::snprintf(buf, len,
-                  "mfma_i32_32x32x32_i8 nt=%d ks=%d nbg=%d stage_j=%d tile_t=%d waves=%d tiles_per_chunk=%d nchunks=%d "
+                  "mfma_i32_16x16x64_i8 nt=%d ks=%d nbg=%d stage_j=%d tile_t=%d waves=%d tiles_per_chunk=%d nchunks=%d "
                   "grid=%d lds=%zu",
                   c.nt, c.ks, c.nbg, c.stage_j ? 1 : 0, kTileT, kWaves, c.tiles_per_chunk, c.nchunks, c.grid,
                   c.lds_bytes);

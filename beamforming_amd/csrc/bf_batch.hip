@@ -398,7 +398,7 @@ int bf_batch_describe(const bf_batch *b, char *buf, size_t len) {
   const int last = b->next_block ^ 1; // the block of the last run
   const MfmaConfig &c = run_config(b, last);
   std::snprintf(buf, len,
-                "persistent mfma_i32_32x32x32_i8 batch: nt=%d ks=%d nbg=%d stage_j=%d grid=%d lds=%zu item_tiles=%d%s "
+                "persistent mfma_i32_16x16x64_i8 batch: nt=%d ks=%d nbg=%d stage_j=%d grid=%d lds=%zu item_tiles=%d%s "
                 "items=%d "
                 "segments=%d "
                 "graph=%s",

@@ -37,10 +37,10 @@ BF_HD uint32_t enc(int re, int im) { return ((((uint32_t)re & 0xfu) << 4) | ((ui
 // The constant offsets depend only on (f, b); the accumulators start at their
 // negation, so they end at exactly raw_re / raw_im (exact int32 arithmetic).
 //
-// MFMA v_mfma_i32_32x32x32_i8 operands: each lane holds 16 int8 of one row
-// (E: time and polarization) or column (W: beam). Per lane and K step the
-// 16 k-values cover 8 dishes as four dwords: [re d0-3 | re d4-7 | im d0-3 |
-// im d4-7] on the weight side. On the E side the nibble dwords
+// MFMA v_mfma_i32_16x16x64_i8 operands: lane l holds 16 int8 of row l % 16
+// (E: time sample) or column l % 16 (W: beam), k-values 16 (l / 16) ..
+// + 15 of the 64. Per lane and MFMA the 16 k-values cover 8 dishes as four
+// dwords: [re d0-3 | re d4-7 | im d0-3 | im d4-7] on the weight side. On the E side the nibble dwords
 // u = [lo_a lo_b hi_a hi_b nlo_a nlo_b] (a = dishes 0-3, b = 4-7) give both
 // variants as overlapping register windows: v2 = u[0..3], v1 = u[2..5].
 
@@ -162,15 +162,15 @@ BF_HD void corrections(int sum_ar, int sum_ai, int *corr_re, int *corr_im) {
 // Work decomposition of the MFMA kernel.
 constexpr int kWaves = 8;            // waves per workgroup (2 per SIMD)
 constexpr int kThreads = 64 * kWaves;
-constexpr int kTileT = 16;           // time samples per wave tile (x 2 polarizations = MFMA M)
+constexpr int kTileT = 16;           // time samples per wave tile (MFMA M; one MFMA row set per polarization)
 constexpr int64_t kMaxTimes = int64_t(1) << 34; // longest chunk: time tile indices stay below 2^30
-constexpr int kDishesPerStep = 16;   // dishes per K step (MFMA K = 32 int8)
-constexpr int kBeamTile = 32;        // beams per MFMA tile (MFMA N)
+constexpr int kDishesPerStep = 16;   // dishes per K step; one MFMA (K = 64 int8) takes two steps
+constexpr int kBeamTile = 32;        // beams per beam tile (two 16-beam MFMA column groups)
 
 struct MfmaParams {
   const uint8_t *E; // [T][F][D][2]
   uint8_t *J;       // [B][F][2][T] or [B][F][T][2]
-  const v4i *W;     // [F][KS][NTP][64] weight operands
+  const v4i *W;     // [F][KS / 2][2 NTP][64] weight operands (32-dish steps, 16-beam groups)
   const int *corr;  // [F][BP][2], BP = 32 NTP
   const float *G;   // [F][BP]
   int64_t T;

@@ -5,20 +5,24 @@
 namespace bf {
 
 // ---------------------------------------------------------------------------
-// Weight operands W[f][s][ntp][lane] (16 bytes each): lane = (g, c) holds
-// beam b = 32 ntp + c, dishes 16 s + 8 g .. +7, zero outside B and D.
+// Weight operands W[f][s][g][lane] (16 bytes each) for the 16x16x64 MFMA:
+// step s covers dishes 32 s .. 32 s + 31, g is a 16-beam group (2 NTP of
+// them over the NTP 32-beam tiles), lane = (kg, c) = (lane / 16, lane % 16)
+// holds beam 16 g + c, dishes 32 s + 8 kg .. +7 (Ar, then Ai), zero outside
+// B and D. KS counts 16-dish steps (even).
 __global__ void prep_weights_kernel(const uint8_t *__restrict__ A, v4i *__restrict__ W, int F, int B, int D, int KS,
                                     int NTP) {
-  const int64_t total = (int64_t)F * KS * NTP * 64;
+  const int KQ = KS / 2, NQP = 2 * NTP;
+  const int64_t total = (int64_t)F * KQ * NQP * 64;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
     const int lane = (int)(i & 63);
     int64_t q = i >> 6;
-    const int ntp = (int)(q % NTP);
-    q /= NTP;
-    const int s = (int)(q % KS);
-    const int f = (int)(q / KS);
-    const int b = ntp * kBeamTile + (lane & 31);
-    const int d0 = kDishesPerStep * s + 8 * (lane >> 5);
+    const int g = (int)(q % NQP);
+    q /= NQP;
+    const int s = (int)(q % KQ);
+    const int f = (int)(q / KQ);
+    const int b = g * 16 + (lane & 15);
+    const int d0 = 2 * kDishesPerStep * s + 8 * (lane >> 4);
     uint8_t a[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {

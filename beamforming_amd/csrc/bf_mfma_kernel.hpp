@@ -1,35 +1,39 @@
 // bf_mfma_kernel.hpp -- the gfx950 form-beams kernel (int8 MFMA, exact).
 //
-// GEMM per frequency: rows = (16 time samples x 2 polarizations) per tile,
-// columns = beams (32 per MFMA tile), K = dishes x (re, im). The MFMA is
-// v_mfma_i32_32x32x32_i8, the full-rate int8 matrix instruction on gfx950.
+// GEMM per frequency: rows = time samples of one polarization (16 per MFMA
+// block), columns = beams (16 per MFMA block), K = dishes x (re, im), 32
+// dishes per K step. The MFMA is v_mfma_i32_16x16x64_i8: of the full-rate
+// int8 shapes of gfx950 it sustains the most operations per second at the
+// board's power limit (4.7 POP/s against 4.1 for 32x32x32 in a
+// register-only loop; the kernel runs at that limit).
 //
-// One workgroup = one (frequency, beam group, time chunk):
+// One workgroup = one (frequency, beam group of NT x 32 beams, time chunk):
 //  * the prepared weight operands of the beam group (KS x NT KiB) are staged
 //    once into LDS and shared by the 8 waves (2 per SIMD);
-//  * each wave owns 16-sample time tiles. Lane (h, m) of the E operand is
-//    row m = (polarization m / 16, time m % 16) and k half h; per K step it
-//    loads the 16 packed E bytes of its time and dishes 16 s + 8 h .. +7
-//    (both polarizations; the lane pair m, m + 16 loads the same bytes) into
-//    a register ring running R steps ahead, keeps its polarization with one
-//    byte permute per 4 dishes and splits the nibbles into the two operand
+//  * each wave owns 16-sample time tiles. Per K step it loads the tile's
+//    16 rows x 64 bytes of E (dishes 32 s .. 32 s + 31, both polarizations)
+//    into a register ring running RP steps ahead, with lane quads reading
+//    contiguous 64-byte row segments, and transposes them through a 1 KiB LDS
+//    buffer so that lane (kg, col) = (lane / 16, lane % 16) holds row (time)
+//    col, dishes 8 kg .. 8 kg + 7; it keeps each polarization with one byte
+//    permute per 4 dishes and splits the nibbles into the two operand
 //    variants (see bf_device.hpp);
-//  * per K step and 32-beam tile: one LDS read of the weight operand and two
-//    MFMAs (real and imaginary accumulator);
+//  * per K step, 16-beam group and polarization: one LDS read of the weight
+//    operand (shared by both polarizations) and two MFMAs (real and
+//    imaginary accumulator);
 //  * the int32 accumulators start at minus the per-beam nibble offsets, so
 //    they end at the exact complex sums of cpu.cxx:24-35; the fused epilogue
 //    applies the float gain, the +-7 clamp and round-to-nearest-even
-//    (cpu.cxx:37-39), packs ucomplex4 bytes and swaps halves between lanes
-//    l and l + 32 (v_permlane32_swap), so that each lane holds the 16 bytes
-//    of 16 consecutive times of one (beam, polarization) row;
+//    (cpu.cxx:37-39) and packs ucomplex4 bytes: each accumulator holds 4
+//    consecutive times of one (beam, polarization), one 4-byte J word;
 //  * the 8 waves advance in rounds of 8 consecutive tiles. When the launch
 //    stages J (room in LDS, and the configuration asks for it), each wave
-//    writes its rows into a double-buffered LDS staging area and, after the
-//    round's barrier, the workgroup stores every row's bytes of the round
-//    (8 tiles: 128 bytes per (beam, polarization) row of [b][f][p][t], 256
-//    bytes per beam row of [b][f][t][p]) with whole-line writes; otherwise
-//    each lane stores its pieces directly. The J layout is a template
-//    parameter (JL), so each kernel holds one epilogue.
+//    writes its words into a double-buffered LDS staging area and, after
+//    the round's barrier, the workgroup stores every row's bytes of the
+//    round (8 tiles: 128 bytes per (beam, polarization) row of [b][f][p][t],
+//    256 bytes per beam row of [b][f][t][p]) with whole-line writes;
+//    otherwise each lane stores its words directly. The J layout is a
+//    template parameter (JL), so each kernel holds one epilogue.
 // Rows past T and dishes past D are clamped, always-in-bounds loads: the
 // outputs of rows past T are never stored and the weights of dishes past D
 // are zero. The plan routes gains outside the fast requantisation range
@@ -63,6 +67,7 @@ template <class T> __device__ __forceinline__ BF_GLOBAL T *gstore_ptr(void *p) {
 typedef v4i v4i_u __attribute__((aligned(1)));
 typedef int v2i __attribute__((ext_vector_type(2)));
 typedef v2i v2i_u __attribute__((aligned(1)));
+typedef uint32_t u32_u __attribute__((aligned(1)));
 
 // Stores 16 bytes (w0..w3, little-endian) at dst, keeping only bytes i < n.
 __device__ __forceinline__ void store_row16(uint8_t *dst, uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, int n) {
@@ -114,11 +119,16 @@ BF_HD constexpr size_t mfma_lds_bytes(int nt, int ks) {
 template <int NT, int KS>
 __device__ __forceinline__ void stage_weights(const v4i *W, const int *corr, const float *G, int NTP, int wrow, int bg,
                                               v4i *Wl, int2 *Cl, float *Gl) {
-  const v4i *src = W + ((size_t)wrow * KS * NTP + (size_t)bg * NT) * 64;
-  for (int i = threadIdx.x; i < KS * NT * 64; i += kThreads) {
-    const int s = i / (NT * 64);
-    const int rem = i - s * (NT * 64);
-    Wl[i] = src[(size_t)s * NTP * 64 + rem];
+  // W [row][KS / 2][2 NTP][64]: per 32-dish step, the 16-beam groups of all
+  // beam tiles; the workgroup's 2 NT groups of every step go to LDS as
+  // Wl [KS / 2][2 NT][64].
+  constexpr int KQ = KS / 2, NQ = 2 * NT;
+  const int NQP = 2 * NTP;
+  const v4i *src = W + ((size_t)wrow * KQ * NQP + (size_t)bg * NQ) * 64;
+  for (int i = threadIdx.x; i < KQ * NQ * 64; i += kThreads) {
+    const int s = i / (NQ * 64);
+    const int rem = i - s * (NQ * 64);
+    Wl[i] = src[(size_t)s * NQP * 64 + rem];
   }
   const size_t g0 = (size_t)wrow * NTP * 32 + (size_t)bg * NT * 32;
   for (int i = threadIdx.x; i < NT * 32; i += kThreads) {
@@ -148,42 +158,63 @@ __device__ __forceinline__ void store_piece(uint8_t *dst, const v4i v, int n, bo
     store_row16(dst, (uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z, (uint32_t)v.w, n);
 }
 
+// Stores the 4 bytes of w (little-endian) at dst, keeping bytes i < n.
+__device__ __forceinline__ void store_row4(uint8_t *dst, uint32_t w, int n, bool whole) {
+  if (whole || n >= 4) {
+    *gstore_ptr<u32_u>(dst) = w;
+  } else {
+    BF_GLOBAL uint8_t *d = gstore_ptr<uint8_t>(dst);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      if (i < n) d[i] = (uint8_t)(w >> (8 * i));
+  }
+}
+
 // The tile pipeline of a workgroup: tiles tile_first .. tile_end - 1 of one
 // job, in rounds of kWaves consecutive tiles (tile tile_first + kWaves r +
 // wave in round r), against the weights staged in LDS (Wl, Cl, Gl), using
 // the wave's transpose buffer xw. Sl is the J staging area (NULL: direct
 // stores). All waves of the workgroup call this with the same arguments.
+//
+// MFMA v_mfma_i32_16x16x64_i8: per K step of 32 dishes, for each
+// polarization q (a 16-row block of the tile: its 16 times) and each 16-beam
+// column group, two MFMAs (real and imaginary accumulator). Lane (kg, col) =
+// (lane / 16, lane % 16) holds, for the E operand, row (time) col and dishes
+// 8 kg .. 8 kg + 7 of the step; for the weight operand, beam col of the group
+// and the same dishes; its accumulators hold rows 4 kg .. 4 kg + 3 (times) of
+// column col (beam). At the board's power limit this shape sustains more
+// int8 operations per second than 32x32x32 (4.7 against 4.1 POP/s in a
+// register-only loop), with the same operand bytes per lane and half the
+// accumulator bytes per multiply.
 template <int NT, int KS, bool DT, int JL>
 __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile_first, const int64_t tile_end,
                                           const v4i *Wl, const int2 *Cl, const float *Gl, v4i *xw, v4i *Sl) {
   static_assert(NT >= 1 && NT <= 4, "32-beam tiles per workgroup");
-  static_assert(KS >= 2 && KS <= 64 && KS % 2 == 0, "K steps (pairs)");
-  constexpr int KP = KS / 2;                  // K-step pairs (one E load each)
-  constexpr int RP0 = 8;                      // E prefetch distance in pairs
-  constexpr int RP = RP0 < KP ? RP0 : KP;
-  static_assert(KP % RP == 0, "ring slots must line up across tiles");
+  static_assert(KS >= 2 && KS <= 64 && KS % 2 == 0, "K steps of 16 dishes, in pairs");
+  constexpr int KQ = KS / 2;                 // K steps of 32 dishes (one E load each)
+  constexpr int NQ = 2 * NT;                 // 16-beam column groups
+  constexpr int RP0 = NT >= 3 ? 4 : 8;       // E prefetch distance in steps (fewer registers at NT >= 3)
+  constexpr int RP = RP0 < KQ ? RP0 : KQ;
+  static_assert(KQ % RP == 0, "ring slots must line up across tiles");
   constexpr int64_t step = kWaves;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t rounds = tile_end > tile_first ? (tile_end - tile_first + step - 1) / step : 0;
   int64_t tile = tile_first + wave;
-  // Every 16-byte J piece is complete (uniform).
+  // Every J piece is complete (uniform).
   const bool fast_store = (job.T & 15) == 0;
 
   const int lane = threadIdx.x & 63;
-  const int h = lane >> 5;          // k half (operands); row half (accumulators)
-  const int col = lane & 31;        // beam column of the accumulators
-  const int pm = (lane >> 4) & 1;   // polarization of this lane's E row
-  const int tm = lane & 15;         // time of this lane's E row within the tile
-  const uint32_t sel = pm ? 0x07050301u : 0x06040200u;
+  const int col = lane & 15; // E row (time) of the operand; beam column of the weights and accumulators
+  const int kg = lane >> 4;  // dish group of the operands; time group 4 kg .. 4 kg + 3 of the accumulators
 
   // E loads, fully coalesced. Row t of this frequency starts at
-  // E + (t F + f) D 2. For K-step pair (2j, 2j + 1) a tile needs bytes
-  // 64 j .. 64 j + 63 of its 16 rows: lane l loads the 16-byte piece l & 3
-  // of row l >> 2, so every lane quad reads one contiguous 64-byte segment.
-  // The wave then transposes through its 1 KiB LDS buffer: the operand of
-  // step 2j + u for lane (h, pm, tm) is piece 2u + h of row tm (dishes
-  // 16 (2j + u) + 8 h .. +7). Pieces are stored at slot (piece + row / 2) & 3
-  // of their row so that the reads of 8 consecutive rows hit distinct banks.
+  // E + (t F + f) D 2. For K step s a tile needs bytes 64 s .. 64 s + 63 of
+  // its 16 rows: lane l loads the 16-byte piece l & 3 of row l >> 2, so every
+  // lane quad reads one contiguous 64-byte segment. The wave then transposes
+  // through its 1 KiB LDS buffer: the operand of lane (kg, col) is piece kg of
+  // row col (dishes 32 s + 8 kg .. +7, both polarizations). Pieces are stored
+  // at slot (piece + row / 2) & 3 of their row, so that the reads of
+  // consecutive rows spread over the banks.
   // Loads are unconditional and in bounds: rows past T re-read row T - 1
   // (their outputs are computed but never stored) and, with DT, dishes past
   // D read the row's first bytes (their weights are zero, so they add 0).
@@ -191,13 +222,11 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
   const int64_t e_tstride = job.e_tstride;
   const uint8_t *Ef = job.E + ld_piece * 16;
   auto row_ptr = [&](int64_t tl) -> const uint8_t * { return Ef + e_row_index(tl, ld_row, job.T) * e_tstride; };
-  auto load_pair = [&](const uint8_t *p, int j) -> v4i {
-    return *gload_ptr<v4i_u>(p + e_col_offset(j, ld_piece, job.D, DT)); // any alignment
+  auto load_step = [&](const uint8_t *p, int s) -> v4i {
+    return *gload_ptr<v4i_u>(p + e_col_offset(s, ld_piece, job.D, DT)); // any alignment
   };
   const int wr_slot = ld_row * 4 + ((ld_piece + (ld_row >> 1)) & 3);
-  int rd_slot[2];
-#pragma unroll
-  for (int u = 0; u < 2; ++u) rd_slot[u] = tm * 4 + ((2 * u + h + (tm >> 1)) & 3);
+  const int rd_slot = col * 4 + ((kg + (col >> 1)) & 3);
 
   // Nothing issued before this call is waited for inside it: completing it
   // here gives the compiler an empty count of memory operations in flight at
@@ -208,168 +237,164 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
   {
     const uint8_t *p0 = row_ptr(tile);
 #pragma unroll
-    for (int q = 0; q < RP; ++q) ring[q] = load_pair(p0, q);
+    for (int q = 0; q < RP; ++q) ring[q] = load_step(p0, q);
   }
 
   for (int64_t r = 0; r < rounds; ++r, tile += step) {
   if (tile < tile_end) { // wave-uniform
-    v16i acc[NT][2];
+    v4i acc[2][NQ][2];
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-      const int2 c = Cl[nt * 32 + col];
+    for (int qq = 0; qq < NQ; ++qq) {
+      const int2 c = Cl[qq * 16 + col];
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        acc[nt][0][j] = -c.x;
-        acc[nt][1][j] = -c.y;
+      for (int q = 0; q < 2; ++q) {
+        acc[q][qq][0] = v4i{-c.x, -c.x, -c.x, -c.x};
+        acc[q][qq][1] = v4i{-c.y, -c.y, -c.y, -c.y};
       }
     }
     const uint8_t *pcur = row_ptr(tile);
     const uint8_t *pnext = row_ptr(tile + step);
 
     // One K loop in straight-line code, one scheduling region per step,
-    // software pipelined so that no step waits for LDS it has just asked
-    // for: during step s the wave reads the weight operands of step s + 1
-    // and the raw E operand of step s + 2 (staging a new pair into the
-    // transpose buffer when step s + 2 starts one), issues the MFMAs of
-    // step s, then unpacks the E operand of step s + 1 (VALU work grouped
-    // after the MFMAs, which co-executes better than interleaving it).
+    // software pipelined: during step s the wave stages the raw E of step
+    // s + 1 through the transpose buffer and reads it back with the weight
+    // operands of step s + 1, issues the MFMAs of step s, then unpacks the E
+    // operands of step s + 1 (VALU work grouped after the MFMAs).
     {
-      auto operands = [&](v4i e, v4i &v1, v4i &v2) {
-        const v8i uo = unpack_e_pol(e, sel);
-        v1 = op_v1(uo);
-        v2 = op_v2(uo);
-      };
-      // stages pair j into the transpose buffer and refills its ring slot
-      auto stage_pair = [&](int j) {
-        const int q = j % RP;
-        xw[wr_slot] = ring[q]; // the wave's LDS operations complete in order
-        const int j2 = j + RP;
-        ring[q] = j2 >= KP ? load_pair(pnext, j2 - KP) : load_pair(pcur, j2);
-      };
-      // prologue: pair 0 staged, raw E of steps 0 and 1, weights of step 0
-      v4i v1c, v2c, raw1, wc[NT];
-      {
-        stage_pair(0);
-        const v4i raw0 = xw[rd_slot[0]];
-        raw1 = xw[rd_slot[1]];
+      auto operands = [&](v4i e, v4i (&v1)[2], v4i (&v2)[2]) {
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) wc[nt] = Wl[nt * 64 + lane];
+        for (int q = 0; q < 2; ++q) {
+          const v8i u = unpack_e_pol(e, q ? 0x07050301u : 0x06040200u);
+          v1[q] = op_v1(u);
+          v2[q] = op_v2(u);
+        }
+      };
+      // stages step s into the transpose buffer and refills its ring slot
+      auto stage_step = [&](int s) {
+        const int q = s % RP;
+        xw[wr_slot] = ring[q]; // the wave's LDS operations complete in order
+        const int s2 = s + RP;
+        ring[q] = s2 >= KQ ? load_step(pnext, s2 - KQ) : load_step(pcur, s2);
+      };
+      v4i v1c[2], v2c[2], wc[NQ];
+      {
+        stage_step(0);
+        const v4i raw0 = xw[rd_slot];
+#pragma unroll
+        for (int qq = 0; qq < NQ; ++qq) wc[qq] = Wl[qq * 64 + lane];
         operands(raw0, v1c, v2c);
       }
 #pragma unroll
-      for (int s = 0; s < KS; ++s) {
-        v4i wn[NT], raw2 = raw1, v1n = v1c, v2n = v2c;
-        if (s + 1 < KS) {
+      for (int s = 0; s < KQ; ++s) {
+        v4i v1n[2], v2n[2], rawn;
+        if (s + 1 < KQ) {
+          stage_step(s + 1);
+          rawn = xw[rd_slot];
+        }
+        // group by group: the four MFMAs of a 16-beam group, then that
+        // group's weights of the next step into the same registers
 #pragma unroll
-          for (int nt = 0; nt < NT; ++nt) wn[nt] = Wl[((s + 1) * NT + nt) * 64 + lane];
-        }
-        if (s + 2 < KS) {
-          if (((s + 2) & 1) == 0) stage_pair((s + 2) / 2);
-          raw2 = xw[rd_slot[(s + 2) & 1]];
-        }
+        for (int qq = 0; qq < NQ; ++qq) {
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-          acc[nt][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(v1c, wc[nt], acc[nt][0], 0, 0, 0);
-          acc[nt][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(v2c, wc[nt], acc[nt][1], 0, 0, 0);
+          for (int q = 0; q < 2; ++q) {
+            acc[q][qq][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(v1c[q], wc[qq], acc[q][qq][0], 0, 0, 0);
+            acc[q][qq][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(v2c[q], wc[qq], acc[q][qq][1], 0, 0, 0);
+          }
+          if (s + 1 < KQ) wc[qq] = Wl[((s + 1) * NQ + qq) * 64 + lane];
+          __builtin_amdgcn_sched_barrier(0); // keeps the loads behind their group's MFMAs (register pressure)
         }
-        if (s + 1 < KS) operands(raw1, v1n, v2n);
+        if (s + 1 < KQ) operands(rawn, v1n, v2n);
         __builtin_amdgcn_sched_barrier(0);
-        v1c = v1n;
-        v2c = v2n;
-        raw1 = raw2;
+        if (s + 1 < KQ) {
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) wc[nt] = wn[nt];
+          for (int q = 0; q < 2; ++q) {
+            v1c[q] = v1n[q];
+            v2c[q] = v2n[q];
+          }
+        }
       }
     }
 
-    // Epilogue. Accumulator element j of lane (h, col) is output row
-    // 8 (j / 4) + 4 h + j % 4 = (polarization j / 8, time 8 ((j / 4) & 1) +
-    // 4 h + j % 4) of beam column col.
+    // Epilogue. Accumulator element i of lane (kg, col) is time 4 kg + i of
+    // the tile, polarization q, beam col of group qq: one requantised
+    // ucomplex4 byte per (re, im) pair, four consecutive times per word.
     const int64_t t0 = tile * kTileT;
     const int nvalid = job.T - t0 < kTileT ? (int)(job.T - t0) : kTileT;
-    uint32_t wv[NT][4]; // [p0: t 4h.., p0: t 8+4h.., p1: t 4h.., p1: t 8+4h..]
+    // The beam column as the epilogue sees it: the empty asm makes it opaque,
+    // so that the J addresses below are computed here, per tile, instead of
+    // being hoisted out of the tile loop and held in registers through the K
+    // loop (12 64-bit addresses at NT = 3).
+    int cx = col;
+    __asm__ volatile("" : "+v"(cx));
+    uint32_t wv[2][NQ];
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-      const float gain = Gl[nt * 32 + col];
+    for (int qq = 0; qq < NQ; ++qq) {
+      const float gain = Gl[qq * 16 + col];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
+      for (int q = 0; q < 2; ++q) {
         uint32_t b[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-          b[i] = jbyte_bits(requant_bits(gain, acc[nt][0][4 * k + i]), requant_bits(gain, acc[nt][1][4 * k + i]));
-        wv[nt][k] = pack_low_bytes(b[0], b[1], b[2], b[3]);
+          b[i] = jbyte_bits(requant_bits(gain, acc[q][qq][0][i]), requant_bits(gain, acc[q][qq][1][i]));
+        wv[q][qq] = pack_low_bytes(b[0], b[1], b[2], b[3]);
       }
     }
     if constexpr (JL == 0) {
-      // J[b][f][p][t]: v_permlane32_swap exchanges lanes 32..63 of its first
-      // operand with lanes 0..31 of its second. Swapping (p0 t0-3 | p1 t0-3)
-      // pairs and (p0 t8-11 | p1 t8-11) pairs leaves lane h = 0 with the
-      // 16 bytes of polarization 0 and lane h = 1 with those of polarization
-      // 1, in the same register order.
-      v4i row[NT];
-#pragma unroll
-      for (int nt = 0; nt < NT; ++nt) {
-        const auto a = __builtin_amdgcn_permlane32_swap(wv[nt][0], wv[nt][2], false, false);
-        const auto c = __builtin_amdgcn_permlane32_swap(wv[nt][1], wv[nt][3], false, false);
-        row[nt] = v4i{(int)a[0], (int)a[1], (int)c[0], (int)c[1]};
-      }
       if (Sl) {
-        // Row (beam nt * 32 + col, polarization h), 16-byte piece `wave`
-        // of the round's 128 bytes; piece slots are rotated by the row so
-        // that the 8 rows of a lane octet hit distinct banks.
-        v4i *buf = Sl + (size_t)(r & 1) * (NT * 64 * 8);
+        // Row (beam qq * 16 + col, polarization q), bytes 4 kg .. 4 kg + 3 of
+        // the wave's 16-byte piece of the round's 128 bytes; piece slots are
+        // rotated by the row (as the cooperative store below reads them).
+        uint32_t *buf = reinterpret_cast<uint32_t *>(Sl + (size_t)(r & 1) * (NT * 64 * 8));
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-          const int srow = (nt * 32 + col) * 2 + h;
-          buf[srow * 8 + (wave ^ ((srow >> 1) & 7))] = row[nt];
-        }
+        for (int qq = 0; qq < NQ; ++qq)
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const int srow = (qq * 16 + cx) * 2 + q;
+            buf[srow * 32 + (wave ^ ((srow >> 1) & 7)) * 4 + kg] = wv[q][qq];
+          }
       } else {
         // The E loads of the next tile's first steps were issued thousands
         // of cycles ago; completing them here keeps the stores (counted by
         // the same vmcnt) out of every wait of the next tile.
         __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0)
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-          const int b = (job.bg * NT + nt) * 32 + col;
+        for (int qq = 0; qq < NQ; ++qq) {
+          const int b = job.bg * NT * 32 + qq * 16 + cx;
           if (b < job.B)
-            store_piece(job.J + j_row_bfpt(b, job.F, job.f, h, job.T) + t0, row[nt], nvalid, fast_store);
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+              store_row4(job.J + j_row_bfpt(b, job.F, job.f, q, job.T) + t0 + 4 * kg, wv[q][qq], nvalid - 4 * kg,
+                         fast_store);
         }
       }
-    } else if (Sl) {
-      // J[b][f][t][p], staged: interleave the polarizations of the lane's
-      // two groups of 4 times (A: times 4h.., B: times 8 + 4h..), then swap
-      // halves between lanes l and l + 32 so that lane h holds the 16 bytes
-      // of times 8h .. 8h + 7 of its beam (h = 0: its A and lane l + 32's A;
-      // h = 1: lane l - 32's B and its B). Row = beam, 256 bytes per round
-      // (8 tiles x 32 bytes); the 16-byte piece slots are rotated by the row
-      // so that 16 consecutive lanes hit distinct banks.
-      v4i *buf = Sl + (size_t)(r & 1) * (NT * 64 * 8);
-#pragma unroll
-      for (int nt = 0; nt < NT; ++nt) {
-        const uint32_t alo = perm_bytes(wv[nt][2], wv[nt][0], 0x05010400u);
-        const uint32_t ahi = perm_bytes(wv[nt][2], wv[nt][0], 0x07030602u);
-        const uint32_t blo = perm_bytes(wv[nt][3], wv[nt][1], 0x05010400u);
-        const uint32_t bhi = perm_bytes(wv[nt][3], wv[nt][1], 0x07030602u);
-        const auto lo = __builtin_amdgcn_permlane32_swap(alo, blo, false, false);
-        const auto hi = __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false);
-        const int srow = nt * 32 + col;
-        buf[srow * 16 + ((2 * wave + h) ^ (srow & 15))] = v4i{(int)lo[0], (int)hi[0], (int)lo[1], (int)hi[1]};
-      }
     } else {
-      __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0), as above
+      // J[b][f][t][p]: interleave the polarizations of the lane's 4 times:
+      // 8 bytes = times 4 kg .. 4 kg + 3 x (p0, p1) of beam qq * 16 + col.
+      uint32_t lo[NQ], hi[NQ];
 #pragma unroll
-      for (int nt = 0; nt < NT; ++nt) {
-        const int b = (job.bg * NT + nt) * 32 + col;
-        if (b < job.B) {
-          // J[b][f][t][p]: interleave the polarizations of 4 times per half.
-          uint8_t *row = job.J + j_row_bftp(b, job.F, job.f, job.T) + 2 * t0;
+      for (int qq = 0; qq < NQ; ++qq) {
+        lo[qq] = perm_bytes(wv[1][qq], wv[0][qq], 0x05010400u);
+        hi[qq] = perm_bytes(wv[1][qq], wv[0][qq], 0x07030602u);
+      }
+      if (Sl) {
+        // Row = beam, 256 bytes per round (8 tiles x 32 bytes); the wave's
+        // 32 bytes are 16-byte slots 2 wave and 2 wave + 1, rotated by the
+        // row (as the cooperative store below reads them).
+        v2i *buf = reinterpret_cast<v2i *>(Sl + (size_t)(r & 1) * (NT * 64 * 8));
 #pragma unroll
-          for (int k = 0; k < 2; ++k) {
-            const int tt = 8 * k + 4 * h; // first time of this group in the tile
-            const uint32_t lo = perm_bytes(wv[nt][2 + k], wv[nt][k], 0x05010400u);
-            const uint32_t hi = perm_bytes(wv[nt][2 + k], wv[nt][k], 0x07030602u);
-            store_row8(row + 2 * tt, lo, hi, 2 * (nvalid - tt));
-          }
+        for (int qq = 0; qq < NQ; ++qq) {
+          const int srow = qq * 16 + cx;
+          const int slot = (2 * wave + (kg >> 1)) ^ (srow & 15);
+          buf[srow * 32 + slot * 2 + (kg & 1)] = v2i{(int)lo[qq], (int)hi[qq]};
+        }
+      } else {
+        __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0), as above
+#pragma unroll
+        for (int qq = 0; qq < NQ; ++qq) {
+          const int b = job.bg * NT * 32 + qq * 16 + cx;
+          if (b < job.B)
+            store_row8(job.J + j_row_bftp(b, job.F, job.f, job.T) + 2 * (t0 + 4 * kg), lo[qq], hi[qq],
+                       2 * (nvalid - 4 * kg));
         }
       }
     }

@@ -69,7 +69,7 @@ struct bf_plan {
   int wgs_per_cu = 1;      // resident MFMA workgroups per CU
   uint8_t *A_dev = nullptr; // [F][B][D] as given
   float *G_dev = nullptr;   // [F][B] as given
-  bf::v4i *W = nullptr;     // [F][KS][NTP][64] weight operands
+  bf::v4i *W = nullptr;     // [F][KS / 2][2 NTP][64] weight operands (32-dish steps, 16-beam groups)
   int *corr = nullptr;      // [F][BP][2]
   float *Gp = nullptr;      // [F][BP]
   bf::HostPath *host = nullptr; // bf_form_beams_host resources (made on first use)

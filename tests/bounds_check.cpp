@@ -10,11 +10,12 @@
 // for both J layouts, with and without the LDS staging area, with and
 // without a dish tail, all beam tilings (NT = 1..4) and grids. The loop
 // structure restates run_tiles in bf_mfma_kernel.hpp: its prologue loads the
-// first RP K-step pairs of the wave's first tile (even for a wave with no
-// tile), each active tile loads its own pairs RP..KP-1 and the first RP pairs
+// first RP 32-dish K steps of the wave's first tile (even for a wave with no
+// tile), each active tile loads its own steps RP..KQ-1 and the first RP steps
 // of the wave's next tile (tile + 8, active or not), and each round either
-// stores the wave's pieces directly or, when staged, stores the whole
-// round's rows cooperatively.
+// stores the wave's words directly (4 bytes per lane, polarization and
+// 16-beam group of [b][f][p][t]; 8 bytes per lane and group of [b][f][t][p])
+// or, when staged, stores the whole round's rows cooperatively.
 // Prints "ok" and exits 0 when every check passes.
 #include "bf_items.hpp"
 #include "bf_mfma_kernel.hpp"
@@ -57,8 +58,10 @@ struct Tiling {
 // tiles [tile_first, tile_end)); e_base is the job's E offset in the buffer.
 void run_job(const Tiling &c, Buffers &buf, int64_t e_base, int64_t e_tstride, int64_t T, int F, int f, int D, int B,
              int bg, int64_t tile_first, int64_t tile_end) {
-  const int KP = c.ks / 2;
-  const int RP = 8 < KP ? 8 : KP;
+  const int KQ = c.ks / 2;            // 32-dish K steps
+  const int RP0 = c.nt >= 3 ? 4 : 8;  // E prefetch distance of run_tiles
+  const int RP = RP0 < KQ ? RP0 : KQ;
+  const int NQ = 2 * c.nt;             // 16-beam groups
   const int64_t step = kWaves;
   const int64_t rounds = tile_end > tile_first ? (tile_end - tile_first + step - 1) / step : 0;
   const bool fast_store = (T & 15) == 0;
@@ -89,10 +92,10 @@ void run_job(const Tiling &c, Buffers &buf, int64_t e_base, int64_t e_tstride, i
       const int64_t tile = tile_first + wave + step * r;
       if (tile >= tile_end) continue;
       for (int lane = 0; lane < 64; ++lane)
-        for (int j = 0; j < KP; ++j) {
+        for (int j = 0; j < KQ; ++j) {
           const int j2 = j + RP;
-          if (j2 >= KP)
-            load(tile + step, lane, j2 - KP);
+          if (j2 >= KQ)
+            load(tile + step, lane, j2 - KQ);
           else
             load(tile, lane, j2);
         }
@@ -100,19 +103,18 @@ void run_job(const Tiling &c, Buffers &buf, int64_t e_base, int64_t e_tstride, i
       const int64_t t0 = tile * kTileT;
       const int nvalid = T - t0 < kTileT ? (int)(T - t0) : kTileT;
       for (int lane = 0; lane < 64; ++lane) {
-        const int h = lane >> 5, col = lane & 31;
-        for (int nt = 0; nt < c.nt; ++nt) {
-          const int b = (bg * c.nt + nt) * 32 + col;
+        const int kg = lane >> 4, col = lane & 15;
+        for (int qq = 0; qq < NQ; ++qq) {
+          const int b = bg * c.nt * 32 + qq * 16 + col;
           if (b >= B) continue;
           if (c.jl == 0) {
-            piece16(j_row_bfpt(b, F, f, h, T) + t0, nvalid);
-          } else {
-            const int64_t row = j_row_bftp(b, F, f, T) + 2 * t0;
-            for (int k = 0; k < 2; ++k) {
-              const int tt = 8 * k + 4 * h;
-              const int n = 2 * (nvalid - tt);
-              store(row + 2 * tt, n >= 8 ? 8 : (n > 0 ? n : 0)); // store_row8
+            for (int q = 0; q < 2; ++q) { // store_row4
+              const int n = nvalid - 4 * kg;
+              store(j_row_bfpt(b, F, f, q, T) + t0 + 4 * kg, fast_store || n >= 4 ? 4 : (n > 0 ? n : 0));
             }
+          } else {
+            const int n = 2 * (nvalid - 4 * kg); // store_row8
+            store(j_row_bftp(b, F, f, T) + 2 * (t0 + 4 * kg), n >= 8 ? 8 : (n > 0 ? n : 0));
           }
         }
       }
