@@ -124,8 +124,13 @@ bool choose_tiling(int64_t ndishes, int64_t nbeams, int forced_nt, MfmaConfig *c
     best_nt = forced_nt;
     best_nbg = (nt_tot + forced_nt - 1) / forced_nt;
   } else {
-    // Fewest padded beam tiles, then fewest beam groups (E passes).
-    for (int nt = 1; nt <= max_nt; ++nt) {
+    // Fewest padded beam tiles, then fewest beam groups (E passes), with at
+    // most 3 tiles per workgroup: at NT = 4 the 128 accumulator registers
+    // leave too few for the operands and the kernel spills (T=8192 F=16
+    // D=512 B=512: 242 us at NT = 4, 217 us at NT = 2 with twice the beam
+    // groups, whose E re-reads hit L2). NT = 4 stays available explicitly.
+    const int auto_nt = max_nt < 3 ? max_nt : 3;
+    for (int nt = 1; nt <= auto_nt; ++nt) {
       const int nbg = (nt_tot + nt - 1) / nt;
       const int padded = nbg * nt;
       if (best_nt == 0 || padded < best_nt * best_nbg || (padded == best_nt * best_nbg && nbg < best_nbg)) {
