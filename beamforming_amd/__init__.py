@@ -229,6 +229,7 @@ def _load():
         "bf_ingest_release": (ctypes.c_int, [vp, ctypes.c_uint64, vp]),
         "bf_ingest_pending": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)]),
         "bf_ingest_sync": (ctypes.c_int, [vp]),
+        "bf_ingest_wait_release": (ctypes.c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -592,6 +593,15 @@ class Ingest:
     def sync(self):
         """Waits until every pushed packet has been read."""
         _check(lib.bf_ingest_sync(self._h))
+
+    def wait_release(self):
+        """Blocks until the oldest pending release has finished and its slot
+        is free; False (at once) when no release is pending."""
+        st = lib.bf_ingest_wait_release(self._h)
+        if st == BF_ERR_NOT_READY:
+            return False
+        _check(st)
+        return True
 
     def pending(self):
         n = ctypes.c_int64()

@@ -35,6 +35,7 @@ struct Slot {
   hipEvent_t released = nullptr; // consumer done (recorded on its stream)
   bool in_use = false;          // filling, waiting to be popped, or popped and not yet released
   bool release_pending = false; // released, event may not have completed
+  uint64_t release_seq = 0;     // order of the release calls (1, 2, ...)
 };
 
 struct Ready {
@@ -129,6 +130,7 @@ struct bf_ingest {
   // dev_size)): later packets inside it skip the pointer-attribute query.
   uintptr_t dev_base = 0;
   size_t dev_size = 0;
+  uint64_t release_count = 0; // bf_ingest_release calls so far
 };
 
 namespace {
@@ -389,6 +391,21 @@ int bf_ingest_release(bf_ingest *g, uint64_t slot_id, void *stream) {
   Slot &s = g->slots[grp][k];
   BF_HIP(hipEventRecord(s.released, reinterpret_cast<hipStream_t>(stream)));
   s.release_pending = true;
+  s.release_seq = ++g->release_count;
+  return BF_OK;
+}
+
+int bf_ingest_wait_release(bf_ingest *g) {
+  if (!g) return fail(BF_ERR_INVALID_ARGUMENT, "NULL argument");
+  // the oldest release whose event may not have completed yet
+  Slot *oldest = nullptr;
+  for (auto &grp : g->slots)
+    for (Slot &s : grp)
+      if (s.in_use && s.release_pending && (!oldest || s.release_seq < oldest->release_seq)) oldest = &s;
+  if (!oldest) return BF_ERR_NOT_READY; // nothing released: only the consumer can free a slot
+  BF_HIP(hipEventSynchronize(oldest->released));
+  oldest->in_use = false;
+  oldest->release_pending = false;
   return BF_OK;
 }
 

@@ -411,6 +411,15 @@ int bf_ingest_pop(bf_ingest *ingest, void *stream, bf_chunk *chunk, uint64_t *sl
  * `stream` so far (the consumer of the chunk) has finished. */
 int bf_ingest_release(bf_ingest *ingest, uint64_t slot_id, void *stream);
 
+/* Blocks until the oldest release (bf_ingest_release) whose consumer work
+ * may still be running has finished, and frees its slot: a receiver whose
+ * push was refused with BF_ERR_NOT_READY calls it and pushes again, instead
+ * of synchronising the whole device, so the copies already queued keep the
+ * GPU busy meanwhile. Returns BF_ERR_NOT_READY at once when no release is
+ * pending (only the consumer can free a slot then). No counterpart in the
+ * reference, which has no ingest. */
+int bf_ingest_wait_release(bf_ingest *ingest);
+
 /* Samples in the chunks being filled and complete chunks not yet popped. */
 int bf_ingest_pending(const bf_ingest *ingest, int64_t *filled_samples, int *ready_chunks);
 

@@ -155,3 +155,57 @@ def test_push_many_and_partial_refusal():
         for i, (ch, data) in enumerate(got):
             kk = i // (F // G)
             assert np.array_equal(data, E[kk * TC:(kk + 1) * TC, ch:ch + G].reshape(-1)), f"chunk {i}"
+
+
+def test_wait_release_frees_the_oldest_slot():
+    # two ring slots, both chunk times filled and popped: a push into a third
+    # chunk time is refused until a release completes; wait_release blocks on
+    # the oldest release only, after which the push goes through. With no
+    # release pending it returns False at once.
+    F, D, G, TC = 8, 64, 2, 64
+    total = 4 * TC
+    E = bo.fill_uniform(total * F * D * 2, 76).reshape(total, F, D * 2)
+    dE = torch.from_numpy(E.reshape(-1)).cuda()
+    row = F * D * 2
+    ng = F // G
+    with bf.Ingest(F, D, TC, G, ring_slots=2) as ing:
+        assert not ing.wait_release()  # nothing released yet
+        assert ing.push_many([dE.data_ptr(), dE.data_ptr() + TC * row], [TC, TC], False) == 2
+        assert not ing.push(dE.data_ptr() + 2 * TC * row, TC, False)  # both slots held
+        popped = []
+        while True:
+            r = ing.pop(stream())
+            if r is None:
+                break
+            popped.append(r)
+        assert len(popped) == 2 * ng
+        for c, sid in popped:
+            assert c.ntimes == TC
+        # release the first chunk time's slots, the second's stay held
+        for c, sid in popped[:ng]:
+            ing.release(sid, stream())
+        waited = 0
+        while not ing.push(dE.data_ptr() + 2 * TC * row, TC, False):
+            assert ing.wait_release(), "a release is pending"
+            waited += 1
+            assert waited <= ng
+        # the second chunk time's slots are still held: the fourth chunk time
+        # is refused and no release is pending
+        assert not ing.wait_release()
+        assert not ing.push(dE.data_ptr() + 3 * TC * row, TC, False)
+        for c, sid in popped[ng:]:
+            ing.release(sid, stream())
+        while not ing.push(dE.data_ptr() + 3 * TC * row, TC, False):
+            assert ing.wait_release()
+        ing.sync()
+        got = []
+        while True:
+            r = ing.pop(stream())
+            if r is None:
+                break
+            torch.cuda.synchronize()
+            got.append((r[0].channel, device_bytes(r[0].E, TC * G * D * 2)))
+        assert len(got) == 2 * ng
+        for i, (ch, data) in enumerate(got):
+            kk = 2 + i // ng
+            assert np.array_equal(data, E[kk * TC:(kk + 1) * TC, ch:ch + G].reshape(-1)), f"chunk {i}"
