@@ -12,12 +12,12 @@
 //    once into LDS and shared by the 8 waves (2 per SIMD);
 //  * each wave owns 16-sample time tiles. Per K step it loads the tile's
 //    16 rows x 64 bytes of E (dishes 32 s .. 32 s + 31, both polarizations)
-//    into a register ring running RP steps ahead, with lane quads reading
-//    contiguous 64-byte row segments, and transposes them through a 1 KiB LDS
-//    buffer so that lane (kg, col) = (lane / 16, lane % 16) holds row (time)
-//    col, dishes 8 kg .. 8 kg + 7; it keeps each polarization with one byte
-//    permute per 4 dishes and splits the nibbles into the two operand
-//    variants (see bf_device.hpp);
+//    into a register ring running RP steps ahead, each lane (kg, col) =
+//    (lane / 16, lane % 16) loading the 16 bytes of its own operand: row
+//    (time) col, dishes 8 kg .. 8 kg + 7 (the four lanes of a row read its
+//    64-byte segment); it keeps each polarization with one byte permute per
+//    4 dishes and splits the nibbles into the two operand variants (see
+//    bf_device.hpp);
 //  * per K step, 16-beam group and polarization: one LDS read of the weight
 //    operand (shared by both polarizations) and two MFMAs (real and
 //    imaginary accumulator);
@@ -96,14 +96,12 @@ __device__ __forceinline__ void store_row8(uint8_t *dst, uint32_t w0, uint32_t w
 }
 
 // LDS of a workgroup: [KS][NT][64] weight operands, the beam-column nibble
-// offsets and gains of the beam group, one 1 KiB E transpose buffer per wave,
-// then, when the launch stages J (and it fits in the 160 KiB of a CU), the J
+// offsets and gains of the beam group, then, when the launch stages J (and it fits in the 160 KiB of a CU), the J
 // staging area: 2 rounds x (NT x 32 beams x 2 polarizations) rows x 128 bytes
 // ([b][f][p][t]), or 2 rounds x NT x 32 beam rows x 256 bytes ([b][f][t][p]).
 constexpr size_t kLdsPerCu = 160 * 1024;
 BF_HD constexpr size_t mfma_lds_unstaged(int nt, int ks) {
-  return (size_t)ks * nt * 64 * sizeof(v4i) + (size_t)nt * 32 * (sizeof(int2) + sizeof(float)) +
-         (size_t)kWaves * 64 * sizeof(v4i);
+  return (size_t)ks * nt * 64 * sizeof(v4i) + (size_t)nt * 32 * (sizeof(int2) + sizeof(float));
 }
 BF_HD constexpr size_t mfma_lds_staging(int nt) { return (size_t)2 * nt * 64 * 128; }
 BF_HD constexpr bool mfma_stages_j(int nt, int ks) {
@@ -172,9 +170,9 @@ __device__ __forceinline__ void store_row4(uint8_t *dst, uint32_t w, int n, bool
 
 // The tile pipeline of a workgroup: tiles tile_first .. tile_end - 1 of one
 // job, in rounds of kWaves consecutive tiles (tile tile_first + kWaves r +
-// wave in round r), against the weights staged in LDS (Wl, Cl, Gl), using
-// the wave's transpose buffer xw. Sl is the J staging area (NULL: direct
-// stores). All waves of the workgroup call this with the same arguments.
+// wave in round r), against the weights staged in LDS (Wl, Cl, Gl). Sl is
+// the J staging area (NULL: direct stores). All waves of the workgroup call
+// this with the same arguments.
 //
 // MFMA v_mfma_i32_16x16x64_i8: per K step of 32 dishes, for each
 // polarization q (a 16-row block of the tile: its 16 times) and each 16-beam
@@ -188,7 +186,7 @@ __device__ __forceinline__ void store_row4(uint8_t *dst, uint32_t w, int n, bool
 // accumulator bytes per multiply.
 template <int NT, int KS, bool DT, int JL>
 __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile_first, const int64_t tile_end,
-                                          const v4i *Wl, const int2 *Cl, const float *Gl, v4i *xw, v4i *Sl) {
+                                          const v4i *Wl, const int2 *Cl, const float *Gl, v4i *Sl) {
   static_assert(NT >= 1 && NT <= 4, "32-beam tiles per workgroup");
   static_assert(KS >= 2 && KS <= 64 && KS % 2 == 0, "K steps of 16 dishes, in pairs");
   constexpr int KQ = KS / 2;                 // K steps of 32 dishes (one E load each)
@@ -207,26 +205,23 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
   const int col = lane & 15; // E row (time) of the operand; beam column of the weights and accumulators
   const int kg = lane >> 4;  // dish group of the operands; time group 4 kg .. 4 kg + 3 of the accumulators
 
-  // E loads, fully coalesced. Row t of this frequency starts at
-  // E + (t F + f) D 2. For K step s a tile needs bytes 64 s .. 64 s + 63 of
-  // its 16 rows: lane l loads the 16-byte piece l & 3 of row l >> 2, so every
-  // lane quad reads one contiguous 64-byte segment. The wave then transposes
-  // through its 1 KiB LDS buffer: the operand of lane (kg, col) is piece kg of
-  // row col (dishes 32 s + 8 kg .. +7, both polarizations). Pieces are stored
-  // at slot (piece + row / 2) & 3 of their row, so that the reads of
-  // consecutive rows spread over the banks.
+  // E loads straight into the operand layout. Row t of this frequency
+  // starts at E + (t F + f) D 2. For K step s the operand of lane (kg, col)
+  // is bytes 64 s + 16 kg .. + 15 of row col of the tile (dishes 32 s + 8 kg
+  // .. + 7, both polarizations): the four lanes kg = 0..3 of a row read its
+  // whole 64-byte segment within one load instruction. (Loading coalesced
+  // lane quads and transposing through LDS instead costs 3% more time at
+  // full size: the LDS round trip spends more energy than the load saves.)
   // Loads are unconditional and in bounds: rows past T re-read row T - 1
   // (their outputs are computed but never stored) and, with DT, dishes past
   // D read the row's first bytes (their weights are zero, so they add 0).
-  const int ld_row = lane >> 2, ld_piece = lane & 3;
+  const int ld_row = col, ld_piece = kg;
   const int64_t e_tstride = job.e_tstride;
   const uint8_t *Ef = job.E + ld_piece * 16;
   auto row_ptr = [&](int64_t tl) -> const uint8_t * { return Ef + e_row_index(tl, ld_row, job.T) * e_tstride; };
   auto load_step = [&](const uint8_t *p, int s) -> v4i {
     return *gload_ptr<v4i_u>(p + e_col_offset(s, ld_piece, job.D, DT)); // any alignment
   };
-  const int wr_slot = ld_row * 4 + ((ld_piece + (ld_row >> 1)) & 3);
-  const int rd_slot = col * 4 + ((kg + (col >> 1)) & 3);
 
   // Nothing issued before this call is waited for inside it: completing it
   // here gives the compiler an empty count of memory operations in flight at
@@ -256,10 +251,11 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
     const uint8_t *pnext = row_ptr(tile + step);
 
     // One K loop in straight-line code, one scheduling region per step,
-    // software pipelined: during step s the wave stages the raw E of step
-    // s + 1 through the transpose buffer and reads it back with the weight
-    // operands of step s + 1, issues the MFMAs of step s, then unpacks the E
-    // operands of step s + 1 (VALU work grouped after the MFMAs).
+    // software pipelined: during step s the wave takes the raw E of step
+    // s + 1 from the ring (refilling its slot RP steps ahead), issues the
+    // MFMAs of step s group by group, each followed by the read of that
+    // group's weight operand of step s + 1, then unpacks the E operands of
+    // step s + 1 (VALU work grouped after the MFMAs).
     {
       auto operands = [&](v4i e, v4i (&v1)[2], v4i (&v2)[2]) {
 #pragma unroll
@@ -269,17 +265,18 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
           v2[q] = op_v2(u);
         }
       };
-      // stages step s into the transpose buffer and refills its ring slot
+      // takes step s out of the ring (into `staged`) and refills its slot
+      v4i staged;
       auto stage_step = [&](int s) {
         const int q = s % RP;
-        xw[wr_slot] = ring[q]; // the wave's LDS operations complete in order
+        staged = ring[q];
         const int s2 = s + RP;
         ring[q] = s2 >= KQ ? load_step(pnext, s2 - KQ) : load_step(pcur, s2);
       };
       v4i v1c[2], v2c[2], wc[NQ];
       {
         stage_step(0);
-        const v4i raw0 = xw[rd_slot];
+        const v4i raw0 = staged;
 #pragma unroll
         for (int qq = 0; qq < NQ; ++qq) wc[qq] = Wl[qq * 64 + lane];
         operands(raw0, v1c, v2c);
@@ -289,7 +286,7 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
         v4i v1n[2], v2n[2], rawn;
         if (s + 1 < KQ) {
           stage_step(s + 1);
-          rawn = xw[rd_slot];
+          rawn = staged;
         }
         // group by group: the four MFMAs of a 16-beam group, then that
         // group's weights of the next step into the same registers
@@ -448,7 +445,7 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
   extern __shared__ v4i Wl[];
   int2 *Cl = reinterpret_cast<int2 *>(Wl + KS * NT * 64);
   float *Gl = reinterpret_cast<float *>(Cl + NT * 32);
-  v4i *Xl = reinterpret_cast<v4i *>(Gl + NT * 32);
+  v4i *Xl = reinterpret_cast<v4i *>(Gl + NT * 32); // the J staging area, where the launch has one
 
   const int work = work_of_block(blockIdx.x, a.per_xcd);
   if (work >= a.nwork) return; // uniform per workgroup, before any barrier
@@ -456,7 +453,6 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
   stage_weights<NT, KS>(a.W, a.corr, a.G, a.NTP, a.wrow0 + wi.f, wi.bg, Wl, Cl, Gl);
   __syncthreads();
 
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t ntiles = (a.T + kTileT - 1) / kTileT;
   int64_t tile_begin, tile_end;
   chunk_tiles(wi.chunk, a.tiles_per_chunk, ntiles, &tile_begin, &tile_end);
@@ -471,8 +467,8 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
   job.B = a.B;
   job.bg = wi.bg;
   job.jlayout = a.jlayout;
-  v4i *Sl = mfma_stages_j(NT, KS) && a.stage_j ? Xl + kWaves * 64 : nullptr; // compiled out where it cannot fit
-  run_tiles<NT, KS, DT, JL>(job, tile_begin, tile_end, Wl, Cl, Gl, Xl + wave * 64, Sl);
+  v4i *Sl = mfma_stages_j(NT, KS) && a.stage_j ? Xl : nullptr; // compiled out where it cannot fit
+  run_tiles<NT, KS, DT, JL>(job, tile_begin, tile_end, Wl, Cl, Gl, Sl);
 }
 
 // ---------------------------------------------------------------------------
@@ -502,7 +498,7 @@ struct BatchParams {
   const int *work_start;    // item w = segments work_start[w] .. work_start[w + 1] - 1
   const int *nitems;        // [number of work items, queue base, 1: queue / 0: one item per workgroup]
   unsigned *head;           // device: the queue counter
-  const v4i *W;       // channel table [C][KS][NTP][64]
+  const v4i *W;       // channel table [C][KS / 2][2 NTP][64]
   const int *corr;    // [C][BP][2]
   const float *G;     // [C][BP]
   int32_t NTP, D, B, jlayout;
@@ -514,10 +510,9 @@ __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a
   extern __shared__ v4i Wl[];
   int2 *Cl = reinterpret_cast<int2 *>(Wl + KS * NT * 64);
   float *Gl = reinterpret_cast<float *>(Cl + NT * 32);
-  v4i *Xl = reinterpret_cast<v4i *>(Gl + NT * 32);
+  v4i *Xl = reinterpret_cast<v4i *>(Gl + NT * 32); // the J staging area, where the launch has one
   __shared__ int s_item;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  v4i *Sl = mfma_stages_j(NT, KS) && a.stage_j ? Xl + kWaves * 64 : nullptr; // compiled out where it cannot fit
+  v4i *Sl = mfma_stages_j(NT, KS) && a.stage_j ? Xl : nullptr; // compiled out where it cannot fit
   int loaded_channel = -1, loaded_bg = -1;
   const int nitems = a.nitems[0];
   const unsigned base = (unsigned)a.nitems[1];
@@ -557,7 +552,7 @@ __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a
       job.B = a.B;
       job.bg = item.bg;
       job.jlayout = a.jlayout;
-      run_tiles<NT, KS, DT, JL>(job, item.tile_begin, item.tile_end, Wl, Cl, Gl, Xl + wave * 64, Sl);
+      run_tiles<NT, KS, DT, JL>(job, item.tile_begin, item.tile_end, Wl, Cl, Gl, Sl);
       __syncthreads(); // the LDS weights and staging are reused by the next segment
     }
     if (queue) __syncthreads(); // s_item is rewritten by the next pop

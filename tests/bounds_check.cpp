@@ -68,7 +68,7 @@ void run_job(const Tiling &c, Buffers &buf, int64_t e_base, int64_t e_tstride, i
   const int64_t jbytes = (int64_t)buf.jcount.size();
 
   auto load = [&](int64_t tile, int lane, int j) {
-    const int ld_row = lane >> 2, ld_piece = lane & 3;
+    const int ld_row = lane & 15, ld_piece = lane >> 4;
     const int64_t off =
         e_base + 16 * ld_piece + e_row_index(tile, ld_row, T) * e_tstride + e_col_offset(j, ld_piece, D, c.dt);
     expect(off >= 0 && off + 16 <= buf.e_bytes, "E load inside the buffer", off, buf.e_bytes);
