@@ -117,20 +117,19 @@ bool choose_tiling(int64_t ndishes, int64_t nbeams, int forced_nt, MfmaConfig *c
   while (ks < kneed) ks *= 2;
   if (ks > 64) return false;
   const int nt_tot = (int)((nbeams + kBeamTile - 1) / kBeamTile);
-  const int max_nt = ks >= 64 ? 2 : 4; // LDS: ks * nt KiB <= 128 KiB; registers cap nt at 4
+  // LDS: ks * nt KiB of weights <= 128 KiB; registers cap nt at 3 (at 4 the
+  // accumulators alone take 128 of a wave's 256 registers, and both MFMA
+  // forms spill; the int8 form ran slower at 4 than at 2 with twice the beam
+  // groups, 242 against 217 us for T=8192 F=16 D=512 B=512)
+  const int max_nt = ks >= 64 ? 2 : 3;
   int best_nt = 0, best_nbg = 0;
   if (forced_nt > 0) {
     if (forced_nt > max_nt) return false;
     best_nt = forced_nt;
     best_nbg = (nt_tot + forced_nt - 1) / forced_nt;
   } else {
-    // Fewest padded beam tiles, then fewest beam groups (E passes), with at
-    // most 3 tiles per workgroup: at NT = 4 the 128 accumulator registers
-    // leave too few for the operands and the kernel spills (T=8192 F=16
-    // D=512 B=512: 242 us at NT = 4, 217 us at NT = 2 with twice the beam
-    // groups, whose E re-reads hit L2). NT = 4 stays available explicitly.
-    const int auto_nt = max_nt < 3 ? max_nt : 3;
-    for (int nt = 1; nt <= auto_nt; ++nt) {
+    // Fewest padded beam tiles, then fewest beam groups (E passes).
+    for (int nt = 1; nt <= max_nt; ++nt) {
       const int nbg = (nt_tot + nt - 1) / nt;
       const int padded = nbg * nt;
       if (best_nt == 0 || padded < best_nt * best_nbg || (padded == best_nt * best_nbg && nbg < best_nbg)) {
@@ -575,9 +574,9 @@ int bf_plan_describe(const bf_plan *p, char *buf, size_t len) {
   if (p->use_mfma) {
     const MfmaConfig &c = p->cfg;
     std::snprintf(buf, len,
-                  "mfma_i32_16x16x64_i8 nt=%d ks=%d nbg=%d stage_j=%d tile_t=%d waves=%d tiles_per_chunk=%d nchunks=%d "
+                  "%s nt=%d ks=%d nbg=%d stage_j=%d tile_t=%d waves=%d tiles_per_chunk=%d nchunks=%d "
                   "grid=%d lds=%zu",
-                  c.nt, c.ks, c.nbg, c.stage_j ? 1 : 0, kTileT, kWaves, c.tiles_per_chunk, c.nchunks, c.grid,
+                  mfma_name(c.ks), c.nt, c.ks, c.nbg, c.stage_j ? 1 : 0, kTileT, kWaves, c.tiles_per_chunk, c.nchunks, c.grid,
                   c.lds_bytes);
     if (p->has_narrow) {
       const size_t used = std::strlen(buf);

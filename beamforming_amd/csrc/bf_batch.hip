@@ -398,11 +398,11 @@ int bf_batch_describe(const bf_batch *b, char *buf, size_t len) {
   const int last = b->next_block ^ 1; // the block of the last run
   const MfmaConfig &c = run_config(b, last);
   std::snprintf(buf, len,
-                "persistent mfma_i32_16x16x64_i8 batch: nt=%d ks=%d nbg=%d stage_j=%d grid=%d lds=%zu item_tiles=%d%s "
+                "persistent %s batch: nt=%d ks=%d nbg=%d stage_j=%d grid=%d lds=%zu item_tiles=%d%s "
                 "items=%d "
                 "segments=%d "
                 "graph=%s",
-                c.nt, c.ks, c.nbg, c.stage_j ? 1 : 0, run_grid(b, last), c.lds_bytes, b->last_item_tiles,
+                mfma_name(c.ks), c.nt, c.ks, c.nbg, c.stage_j ? 1 : 0, run_grid(b, last), c.lds_bytes, b->last_item_tiles,
                 b->item_tiles ? "" : " (balanced)",
                 b->last_nitems, b->last_segments, b->blk[0].exec || b->blk[1].exec ? "captured" : "none");
   return BF_OK;
@@ -420,11 +420,11 @@ int bf_plan_autotune(bf_plan *p, uint8_t *J, const uint8_t *E, void *stream, int
   TuneChoice best;
   float best_ms = 1e30f;
   int min_ntp = 1 << 30;
-  for (int nt = 1; nt <= 4; ++nt) {
+  for (int nt = 1; nt <= kMaxBeamTiles; ++nt) {
     MfmaConfig t;
     if (choose_tiling(s.ndishes, s.nbeams, nt, &t)) min_ntp = std::min(min_ntp, t.ntp);
   }
-  for (int nt = 1; nt <= 4; ++nt) {
+  for (int nt = 1; nt <= kMaxBeamTiles; ++nt) {
     MfmaConfig t;
     if (!choose_tiling(s.ndishes, s.nbeams, nt, &t)) continue;
     if (3 * t.ntp > 4 * min_ntp) continue; // pads a third more beam tiles than the best tiling
@@ -470,8 +470,9 @@ int bf_plan_autotune(bf_plan *p, uint8_t *J, const uint8_t *E, void *stream, int
 int bf_plan_configure(bf_plan *p, int beam_tiles, int stage_j, double wg_scale) {
   if (!p) return fail(BF_ERR_INVALID_ARGUMENT, "NULL plan");
   if (!p->use_mfma) return fail(BF_ERR_UNSUPPORTED, "the plan does not use the MFMA kernel");
-  if (beam_tiles < 0 || beam_tiles > 4 || (stage_j != 0 && stage_j != 1) || !(wg_scale > 0.0) || wg_scale > 64.0)
-    return fail(BF_ERR_INVALID_ARGUMENT, "beam_tiles in 0..4, stage_j 0 or 1, wg_scale in (0, 64]");
+  if (beam_tiles < 0 || beam_tiles > kMaxBeamTiles || (stage_j != 0 && stage_j != 1) || !(wg_scale > 0.0) ||
+      wg_scale > 64.0)
+    return fail(BF_ERR_INVALID_ARGUMENT, "beam_tiles in 0..3, stage_j 0 or 1, wg_scale in (0, 64]");
   const TuneChoice choice{beam_tiles, wg_scale, stage_j != 0};
   MfmaConfig c;
   if (!choose_mfma_config(p->shape, p->num_cus, &c, &choice))

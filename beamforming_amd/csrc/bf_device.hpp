@@ -16,6 +16,7 @@
 namespace bf {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // ucomplex4 codec (icomplex4.hxx:33-37): offset binary, real part in the high
@@ -78,17 +79,49 @@ BF_HD v8i unpack_e_pol(const v4i e, uint32_t sel) {
 BF_HD v4i op_v1(const v8i u) { return __builtin_shufflevector(u, u, 2, 3, 4, 5); }
 BF_HD v4i op_v2(const v8i u) { return __builtin_shufflevector(u, u, 0, 1, 2, 3); }
 
+// fp8 form of the same arithmetic (v_mfma_scale_f32_16x16x128_f8f6f4, A in
+// E4M3, B in E5M2, fp32 accumulators). The E operand bytes are the nibble
+// values n = 0..15 above, unchanged: as E4M3 bit patterns (exponent bias 7,
+// 3 mantissa bits) 0..7 are the subnormals n 2^-9 and 8..15 the normals
+// (1 + (n - 8) / 8) 2^-6 = n 2^-9, so the byte n is the number n 2^-9 exactly.
+// A weight w in -8..7 is stored as the E5M2 code of w 2^9 (all of them have
+// at most 3 significant bits, so every one is exact). Each product is then
+// the integer n w, every partial sum an integer of magnitude below 2^17, and
+// the fp32 accumulation is exact: the accumulators end at the same raw sums
+// as the int8 MFMA, as floats. Used for K steps of 64 dishes, KS = 4 .. 32
+// (at KS = 64 the longer unrolled loop does not fit the registers).
+BF_HD constexpr bool mfma_fp8(int ks) { return ks >= 4 && ks <= 32; }
+// The MFMA instruction of a K-step count, as plan and batch descriptions name it.
+inline const char *mfma_name(int ks) {
+  return mfma_fp8(ks) ? "mfma_f32_16x16x128_f8f6f4(e4m3 x e5m2)" : "mfma_i32_16x16x64_i8";
+}
+BF_HD uint8_t e5m2_times512(int w) {
+  // |w| 2^9 = m 2^e with m in {1, 1.25, 1.5, 1.75}: exponent field e + 15,
+  // mantissa field 4 (m - 1)
+  const int a = w < 0 ? -w : w;
+  if (a == 0) return 0;
+  int e = 9, m4 = 4 * a; // a 2^9 = (m4 / 4) 2^e, normalised below to m4 in 4..7
+  while (m4 >= 8) {
+    m4 >>= 1;
+    ++e;
+  }
+  return (uint8_t)((w < 0 ? 0x80 : 0) | ((e + 15) << 2) | (m4 - 4));
+}
+
 // Weight operand for one lane: 8 packed A bytes (dishes d0..d0+7, zero-padded
-// by the caller) -> [Ar d0-3 | Ar d4-7 | Ai d0-3 | Ai d4-7] as int8.
-BF_HD v4i pack_weights(const uint8_t a[8]) {
+// by the caller) -> [Ar d0-3 | Ar d4-7 | Ai d0-3 | Ai d4-7] as int8, or with
+// fp8 as the E5M2 codes of 512 times those values.
+BF_HD v4i pack_weights(const uint8_t a[8], bool fp8 = false) {
   uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int q = 0; q < 2; ++q)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const uint32_t x = a[4 * q + i];
-      w[q + 0] |= ((uint32_t)dec_re(x) & 0xffu) << (8 * i);
-      w[q + 2] |= ((uint32_t)dec_im(x) & 0xffu) << (8 * i);
+      const uint32_t re = fp8 ? e5m2_times512(dec_re(x)) : (uint32_t)dec_re(x) & 0xffu;
+      const uint32_t im = fp8 ? e5m2_times512(dec_im(x)) : (uint32_t)dec_im(x) & 0xffu;
+      w[q + 0] |= re << (8 * i);
+      w[q + 2] |= im << (8 * i);
     }
   return v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
 }
@@ -131,8 +164,8 @@ BF_HD float requant_exact(float g, int raw) {
 // rounded to the nearest integer, ties to even; M is even, so the tie goes
 // to the even rint(c), as in rint. The float bits of M + 8 + rint(c) are
 // bits(1.5 * 2^23) + 8 + rint(c), whose low 22 bits are 8 + rint(c).
-BF_HD uint32_t requant_bits(float g, int raw) {
-  const float y = g * (float)raw;
+BF_HD uint32_t requant_bits_f(float g, float raw) {
+  const float y = g * raw;
 #if defined(__HIP_DEVICE_COMPILE__)
   const float c = __builtin_amdgcn_fmed3f(y, -7.0f, 7.0f);
 #else
@@ -140,6 +173,10 @@ BF_HD uint32_t requant_bits(float g, int raw) {
 #endif
   return __builtin_bit_cast(uint32_t, c + 12582920.0f);
 }
+// The same from an int32 raw sum (|raw| < 2^24, so the conversion is exact),
+// and from a raw sum already held exactly as a float.
+BF_HD uint32_t requant_bits(float g, int raw) { return requant_bits_f(g, (float)raw); }
+BF_HD uint32_t requant_bits(float g, float raw) { return requant_bits_f(g, raw); }
 
 // ucomplex4 byte of requant_bits pairs in the low byte: (re + 8) << 4 |
 // (im + 8), i.e. enc(qr, qi) (icomplex4.hxx:33). Higher bytes are garbage.
@@ -166,6 +203,7 @@ constexpr int kTileT = 16;           // time samples per wave tile (MFMA M; one 
 constexpr int64_t kMaxTimes = int64_t(1) << 34; // longest chunk: time tile indices stay below 2^30
 constexpr int kDishesPerStep = 16;   // dishes per K step; one MFMA (K = 64 int8) takes two steps
 constexpr int kBeamTile = 32;        // beams per beam tile (two 16-beam MFMA column groups)
+constexpr int kMaxBeamTiles = 3;     // beam tiles per workgroup (NT), registers permitting
 
 struct MfmaParams {
   const uint8_t *E; // [T][F][D][2]
@@ -200,10 +238,14 @@ BF_HD int64_t e_row_index(int64_t tile, int row, int64_t T) {
   return t < T ? t : T - 1;
 }
 // Byte offset within the row of the piece's load, relative to byte
-// 16 piece. With a dish tail (dt), pieces wholly past D re-read the row's
-// first 16 bytes (their weights are zero).
+// 16 piece. With a dish tail (dt), pieces wholly past D (D is a multiple of
+// 8, so a piece is either wholly inside or wholly past) re-read the row's
+// last 16 bytes instead (their weights are zero): the offset is clamped to
+// e_col_limit, the last offset whose 16 bytes lie inside the row.
+BF_HD int e_col_limit(int piece, int D) { return 2 * D - 16 - 16 * piece; }
+BF_HD int e_col_offset_lim(int j, int lim) { return 64 * j < lim ? 64 * j : lim; }
 BF_HD int64_t e_col_offset(int j, int piece, int D, bool dt) {
-  if (dt) return 32 * j + 8 * piece < D ? 64 * j : -16 * piece;
+  if (dt) return e_col_offset_lim(j, e_col_limit(piece, D));
   return 64 * j;
 }
 // First byte of J row (beam b, frequency f, polarization p) of

@@ -9,7 +9,8 @@ namespace bf {
 // step s covers dishes 32 s .. 32 s + 31, g is a 16-beam group (2 NTP of
 // them over the NTP 32-beam tiles), lane = (kg, c) = (lane / 16, lane % 16)
 // holds beam 16 g + c, dishes 32 s + 8 kg .. +7 (Ar, then Ai), zero outside
-// B and D. KS counts 16-dish steps (even).
+// B and D; int8, or E5M2 codes of 512 times the values where the kernel runs
+// the fp8 MFMA (KS >= 4). KS counts 16-dish steps (even).
 __global__ void prep_weights_kernel(const uint8_t *__restrict__ A, v4i *__restrict__ W, int F, int B, int D, int KS,
                                     int NTP) {
   const int KQ = KS / 2, NQP = 2 * NTP;
@@ -29,7 +30,7 @@ __global__ void prep_weights_kernel(const uint8_t *__restrict__ A, v4i *__restri
       const int d = d0 + k;
       a[k] = (b < B && d < D) ? A[((int64_t)f * B + b) * D + d] : (uint8_t)0x88; // 0x88 decodes to (0, 0)
     }
-    W[i] = pack_weights(a);
+    W[i] = pack_weights(a, mfma_fp8(KS));
   }
 }
 

@@ -1,11 +1,12 @@
-// bf_mfma_kernel.hpp -- the gfx950 form-beams kernel (int8 MFMA, exact).
+// bf_mfma_kernel.hpp -- the gfx950 form-beams kernel (MFMA, exact).
 //
 // GEMM per frequency: rows = time samples of one polarization (16 per MFMA
-// block), columns = beams (16 per MFMA block), K = dishes x (re, im), 32
-// dishes per K step. The MFMA is v_mfma_i32_16x16x64_i8: of the full-rate
-// int8 shapes of gfx950 it sustains the most operations per second at the
-// board's power limit (4.7 POP/s against 4.1 for 32x32x32 in a
-// register-only loop; the kernel runs at that limit).
+// block), columns = beams (16 per MFMA block), K = dishes x (re, im). The
+// MFMA is v_mfma_scale_f32_16x16x128_f8f6f4 (64 dishes per instruction, fp8
+// operands holding the int4 values exactly, fp32 accumulators holding exact
+// integer sums) for 64 .. 512 dishes, and v_mfma_i32_16x16x64_i8 (32 dishes
+// per instruction) for fewer or more: at the board's power limit, where the
+// kernel runs, these small shapes sustain the most operations per joule.
 //
 // One workgroup = one (frequency, beam group of NT x 32 beams, time chunk):
 //  * the prepared weight operands of the beam group (KS x NT KiB) are staged
@@ -18,10 +19,11 @@
 //    64-byte segment); it keeps each polarization with one byte permute per
 //    4 dishes and splits the nibbles into the two operand variants (see
 //    bf_device.hpp);
-//  * per K step, 16-beam group and polarization: one LDS read of the weight
-//    operand (shared by both polarizations) and two MFMAs (real and
-//    imaginary accumulator);
-//  * the int32 accumulators start at minus the per-beam nibble offsets, so
+//  * per K step (fp8: pair of steps) and 16-beam group: one LDS read of the
+//    weight operand (fp8: two), shared by both polarizations, and per
+//    polarization two MFMAs (real and imaginary accumulator);
+//  * the accumulators (int32, or fp32 holding integers exactly) start at
+//    minus the per-beam nibble offsets, so
 //    they end at the exact complex sums of cpu.cxx:24-35; the fused epilogue
 //    applies the float gain, the +-7 clamp and round-to-nearest-even
 //    (cpu.cxx:37-39) and packs ucomplex4 bytes: each accumulator holds 4
@@ -174,22 +176,30 @@ __device__ __forceinline__ void store_row4(uint8_t *dst, uint32_t w, int n, bool
 // the J staging area (NULL: direct stores). All waves of the workgroup call
 // this with the same arguments.
 //
-// MFMA v_mfma_i32_16x16x64_i8: per K step of 32 dishes, for each
-// polarization q (a 16-row block of the tile: its 16 times) and each 16-beam
-// column group, two MFMAs (real and imaginary accumulator). Lane (kg, col) =
-// (lane / 16, lane % 16) holds, for the E operand, row (time) col and dishes
-// 8 kg .. 8 kg + 7 of the step; for the weight operand, beam col of the group
-// and the same dishes; its accumulators hold rows 4 kg .. 4 kg + 3 (times) of
-// column col (beam). At the board's power limit this shape sustains more
-// int8 operations per second than 32x32x32 (4.7 against 4.1 POP/s in a
-// register-only loop), with the same operand bytes per lane and half the
-// accumulator bytes per multiply.
+// MFMA forms. int8, v_mfma_i32_16x16x64_i8: per K step of 32 dishes, for
+// each polarization q (a 16-row block of the tile: its 16 times) and each
+// 16-beam column group, two MFMAs (real and imaginary accumulator). Lane
+// (kg, col) = (lane / 16, lane % 16) holds, for the E operand, row (time)
+// col and dishes 8 kg .. 8 kg + 7 of the step; for the weight operand, beam
+// col of the group and the same dishes; its accumulators hold rows
+// 4 kg .. 4 kg + 3 (times) of column col (beam). fp8 (KS = 4 .. 32),
+// v_mfma_scale_f32_16x16x128_f8f6f4 with unit scales: the same operands of
+// two consecutive steps side by side (E bytes read as E4M3, weights as
+// E5M2 codes of 512 w; exact, see mfma_fp8), one MFMA per pair of steps and
+// fp32 accumulators. At the board's power limit the smaller shapes sustain
+// more operations per joule: i8 16x16x64 4.7 POP/s against 4.1 for
+// 32x32x32 in a register-only loop, and the fp8 form draws 6% less power
+// than i8 16x16x64 for 3% more operations (1207 W at 4.89 POP/s against
+// 1283 W at 4.73, same loop); the fp8 form also needs no int-to-float
+// conversion in the epilogue.
 template <int NT, int KS, bool DT, int JL>
 __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile_first, const int64_t tile_end,
                                           const v4i *Wl, const int2 *Cl, const float *Gl, v4i *Sl) {
-  static_assert(NT >= 1 && NT <= 4, "32-beam tiles per workgroup");
+  static_assert(NT >= 1 && NT <= kMaxBeamTiles, "32-beam tiles per workgroup");
   static_assert(KS >= 2 && KS <= 64 && KS % 2 == 0, "K steps of 16 dishes, in pairs");
   constexpr int KQ = KS / 2;                 // K steps of 32 dishes (one E load each)
+  constexpr bool F8 = mfma_fp8(KS);          // fp8 MFMA over pairs of steps (64 dishes)
+  typedef typename std::conditional<F8, v4f, v4i>::type Acc;
   constexpr int NQ = 2 * NT;                 // 16-beam column groups
   constexpr int RP0 = NT >= 3 ? 4 : 8;       // E prefetch distance in steps (fewer registers at NT >= 3)
   constexpr int RP = RP0 < KQ ? RP0 : KQ;
@@ -219,8 +229,13 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
   const int64_t e_tstride = job.e_tstride;
   const uint8_t *Ef = job.E + ld_piece * 16;
   auto row_ptr = [&](int64_t tl) -> const uint8_t * { return Ef + e_row_index(tl, ld_row, job.T) * e_tstride; };
+  // With a dish tail the per-lane clamp limit is made opaque in every tile
+  // (below), so that the clamped offsets are computed at each load instead of
+  // being hoisted out of the tile loop and held in registers.
+  int e_lim = e_col_limit(ld_piece, job.D);
   auto load_step = [&](const uint8_t *p, int s) -> v4i {
-    return *gload_ptr<v4i_u>(p + e_col_offset(s, ld_piece, job.D, DT)); // any alignment
+    const int off = DT ? e_col_offset_lim(s, e_lim) : 64 * s;
+    return *gload_ptr<v4i_u>(p + off); // any alignment
   };
 
   // Nothing issued before this call is waited for inside it: completing it
@@ -237,14 +252,20 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
 
   for (int64_t r = 0; r < rounds; ++r, tile += step) {
   if (tile < tile_end) { // wave-uniform
-    v4i acc[2][NQ][2];
+    if constexpr (DT) __asm__ volatile("" : "+v"(e_lim));
+    Acc acc[2][NQ][2];
 #pragma unroll
     for (int qq = 0; qq < NQ; ++qq) {
       const int2 c = Cl[qq * 16 + col];
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
-        acc[q][qq][0] = v4i{-c.x, -c.x, -c.x, -c.x};
-        acc[q][qq][1] = v4i{-c.y, -c.y, -c.y, -c.y};
+        if constexpr (F8) {
+          acc[q][qq][0] = v4f{(float)-c.x, (float)-c.x, (float)-c.x, (float)-c.x};
+          acc[q][qq][1] = v4f{(float)-c.y, (float)-c.y, (float)-c.y, (float)-c.y};
+        } else {
+          acc[q][qq][0] = v4i{-c.x, -c.x, -c.x, -c.x};
+          acc[q][qq][1] = v4i{-c.y, -c.y, -c.y, -c.y};
+        }
       }
     }
     const uint8_t *pcur = row_ptr(tile);
@@ -273,6 +294,77 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
         const int s2 = s + RP;
         ring[q] = s2 >= KQ ? load_step(pnext, s2 - KQ) : load_step(pcur, s2);
       };
+      if constexpr (F8) {
+        // Steps of 64 dishes: the operands of lane (kg, col) are the two
+        // 32-dish operands of steps 2 h and 2 h + 1 side by side (32 bytes),
+        // against the two weight operands of the group side by side.
+        constexpr int KH = KQ / 2;
+        auto cat = [](v4i a, v4i b) -> v8i { return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7); };
+        auto operands8 = [&](v4i e0, v4i e1, v8i (&a1)[2], v8i (&a2)[2]) {
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const v8i u0 = unpack_e_pol(e0, q ? 0x07050301u : 0x06040200u);
+            const v8i u1 = unpack_e_pol(e1, q ? 0x07050301u : 0x06040200u);
+            a1[q] = cat(op_v1(u0), op_v1(u1));
+            a2[q] = cat(op_v2(u0), op_v2(u1));
+          }
+        };
+        // Weight operands in flight: group g = h NQ + qq is read into slot
+        // qq % WF right after the MFMAs of group g - WF.
+        constexpr int WF = NT == 3 ? NQ / 2 : NQ;
+        static_assert(NQ % WF == 0, "weight slot of group g is qq % WF");
+        constexpr int NGF = KH * NQ;
+        auto wload = [&](int g, v4i &w0, v4i &w1) {
+          const int hh = g / NQ, qq = g - hh * NQ;
+          w0 = Wl[(2 * hh * NQ + qq) * 64 + lane];
+          w1 = Wl[((2 * hh + 1) * NQ + qq) * 64 + lane];
+        };
+        v8i a1c[2], a2c[2];
+        v4i wc0[WF], wc1[WF];
+        {
+          stage_step(0);
+          const v4i r0 = staged;
+          stage_step(1);
+          const v4i r1 = staged;
+#pragma unroll
+          for (int g = 0; g < WF; ++g) wload(g, wc0[g], wc1[g]);
+          operands8(r0, r1, a1c, a2c);
+        }
+#pragma unroll
+        for (int h = 0; h < KH; ++h) {
+          v8i a1n[2], a2n[2];
+          v4i rn0, rn1;
+          if (h + 1 < KH) {
+            stage_step(2 * h + 2);
+            rn0 = staged;
+            stage_step(2 * h + 3);
+            rn1 = staged;
+          }
+#pragma unroll
+          for (int qq = 0; qq < NQ; ++qq) {
+            const int g = h * NQ + qq;
+            const v8i w = cat(wc0[qq % WF], wc1[qq % WF]);
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+              acc[q][qq][0] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a1c[q], w, acc[q][qq][0], 0, 1, 0, 127,
+                                                                               0, 127);
+              acc[q][qq][1] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a2c[q], w, acc[q][qq][1], 0, 1, 0, 127,
+                                                                               0, 127);
+            }
+            if (g + WF < NGF) wload(g + WF, wc0[qq % WF], wc1[qq % WF]);
+            __builtin_amdgcn_sched_barrier(0);
+          }
+          if (h + 1 < KH) operands8(rn0, rn1, a1n, a2n);
+          __builtin_amdgcn_sched_barrier(0);
+          if (h + 1 < KH) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+              a1c[q] = a1n[q];
+              a2c[q] = a2n[q];
+            }
+          }
+        }
+      } else {
       v4i v1c[2], v2c[2], wc[NQ];
       {
         stage_step(0);
@@ -310,6 +402,7 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
           }
         }
       }
+      } // int8
     }
 
     // Epilogue. Accumulator element i of lane (kg, col) is time 4 kg + i of

@@ -12,8 +12,6 @@ MfmaKernel mfma_kernel_ks16(int nt, bool dish_tail, int jl) {
     return mfma_instance<2, 16>(dish_tail, jl);
   case 3:
     return mfma_instance<3, 16>(dish_tail, jl);
-  case 4:
-    return mfma_instance<4, 16>(dish_tail, jl);
   default:
     return nullptr;
   }
@@ -27,8 +25,6 @@ BatchKernel batch_kernel_ks16(int nt, bool dish_tail, int jl) {
     return batch_instance<2, 16>(dish_tail, jl);
   case 3:
     return batch_instance<3, 16>(dish_tail, jl);
-  case 4:
-    return batch_instance<4, 16>(dish_tail, jl);
   default:
     return nullptr;
   }

@@ -227,7 +227,7 @@ int bf_batch_run_graph(bf_batch *batch, const bf_chunk *chunks, int nchunks, voi
 int bf_batch_describe(const bf_batch *batch, char *buf, size_t len);
 
 /* Sets the kernel configuration explicitly, as the autotuner does after
- * timing: beam_tiles 32-beam tiles per workgroup (1..4, 0 = automatic),
+ * timing: beam_tiles 32-beam tiles per workgroup (1..3, 0 = automatic),
  * stage_j 1 to stage J rows in LDS where the layout and the LDS allow it,
  * wg_scale times the default number of workgroups. Work already enqueued
  * with the plan must have finished. Synchronous. */

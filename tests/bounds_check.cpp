@@ -8,7 +8,7 @@
 //  * every J store lies inside the J buffer of its launch or chunk,
 //  * every J byte is written exactly once (nothing missed, nothing twice),
 // for both J layouts, with and without the LDS staging area, with and
-// without a dish tail, all beam tilings (NT = 1..4) and grids. The loop
+// without a dish tail, all beam tilings (NT = 1..3) and grids. The loop
 // structure restates run_tiles in bf_mfma_kernel.hpp: its prologue loads the
 // first RP 32-dish K steps of the wave's first tile (even for a wave with no
 // tile), each active tile loads its own steps RP..KQ-1 and the first RP steps
@@ -146,7 +146,7 @@ Tiling random_tiling(int D, int B, int jl) {
   c.ks = 2;
   while (c.ks < kneed) c.ks *= 2;
   c.dt = c.ks * kDishesPerStep != D;
-  const int max_nt = c.ks >= 64 ? 2 : 4;
+  const int max_nt = c.ks >= 64 ? 2 : kMaxBeamTiles;
   c.nt = (int)pick(1, max_nt);
   const int nt_tot = (B + kBeamTile - 1) / kBeamTile;
   c.nbg = (nt_tot + c.nt - 1) / c.nt;

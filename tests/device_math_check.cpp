@@ -1,6 +1,7 @@
 // Host checks of the per-lane arithmetic of the gfx950 kernels
 // (beamforming_amd/csrc/bf_device.hpp), compiled for the CPU: the byte
-// permute, nibble unpack, operand windows, weight packing, nibble-offset
+// permute, nibble unpack, operand windows, weight packing (int8 and fp8),
+// the fp8 reading of the operands, nibble-offset
 // corrections, the fast and exact requantisation and the output packing,
 // each against the oracle's codec (icomplex4.hxx:22-40), its requantisation
 // (cpu.cxx:37-39) and the textbook complex product (cpu.cxx:24-35).
@@ -101,6 +102,54 @@ void check_operands() {
   }
 }
 
+// OCP 8-bit float formats (the operands of v_mfma_scale_f32_16x16x128_f8f6f4
+// with format codes 0 and 1), decoded from the bit fields: E4M3 (exponent
+// bias 7, 3 mantissa bits, subnormals at exponent 0) and E5M2 (bias 15, 2
+// mantissa bits). NaN and infinity codes are never produced here.
+double dec_e4m3(uint8_t x) {
+  const int s = x >> 7, e = (x >> 3) & 15, m = x & 7;
+  const double v = e == 0 ? std::ldexp(m / 8.0, 1 - 7) : std::ldexp(1.0 + m / 8.0, e - 7);
+  return s ? -v : v;
+}
+double dec_e5m2(uint8_t x) {
+  const int s = x >> 7, e = (x >> 2) & 31, m = x & 3;
+  const double v = e == 0 ? std::ldexp(m / 4.0, 1 - 15) : std::ldexp(1.0 + m / 4.0, e - 15);
+  return s ? -v : v;
+}
+
+// The fp8 form: the E operand bytes (nibble values 0..15) read as E4M3 and
+// the weight bytes of pack_weights(a, true) read as E5M2; every product is
+// an integer and the dot product equals the int8 form's, so the same
+// corrections give the exact raw sums.
+void check_fp8_operands() {
+  for (int n = 0; n < 16; ++n) expect(dec_e4m3((uint8_t)n) * 512.0 == n, "e4m3 of a nibble byte", n, 0);
+  for (int w = -8; w <= 7; ++w)
+    expect(dec_e5m2(e5m2_times512(w)) == 512.0 * w, "e5m2 of 512 w", (long long)dec_e5m2(e5m2_times512(w)), 512 * w);
+  for (int trial = 0; trial < 20000; ++trial) {
+    uint8_t e[16], a[8];
+    for (int i = 0; i < 16; ++i) e[i] = (uint8_t)next();
+    for (int i = 0; i < 8; ++i) a[i] = (uint8_t)next();
+    if (trial < 256) {
+      std::memset(e, trial, sizeof e);
+      std::memset(a, trial & 1 ? 0x00 : 0xff, sizeof a);
+    }
+    uint32_t w[4];
+    std::memcpy(w, e, 16);
+    const v4i ev{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+    const v4i W8 = pack_weights(a), WF = pack_weights(a, true);
+    for (int p = 0; p < 2; ++p) {
+      const v8i u = unpack_e_pol(ev, p ? 0x07050301u : 0x06040200u);
+      for (int variant = 0; variant < 2; ++variant) {
+        const v4i op = variant ? op_v2(u) : op_v1(u);
+        double f = 0;
+        for (int i = 0; i < 16; ++i) f += dec_e4m3((uint8_t)byte_of(op, i)) * dec_e5m2((uint8_t)byte_of(WF, i));
+        const int want = dot16(op, W8);
+        expect(f == (double)want, "fp8 dot product", (long long)f, want);
+      }
+    }
+  }
+}
+
 // Gains the fast path takes: finite, |G raw| < 2^30 for |raw| <= 2^17.
 void check_requant_gain(float g, int raw_lo, int raw_hi, int stride) {
   for (int raw = raw_lo; raw <= raw_hi; raw += stride) {
@@ -162,6 +211,7 @@ int main() {
   check_codec();
   check_perm();
   check_operands();
+  check_fp8_operands();
   check_requant();
   check_packing();
   if (failures) {
