@@ -418,17 +418,28 @@ int launch_mfma(const bf_plan *p, const MfmaConfig &c, uint8_t *J, const uint8_t
   return BF_OK;
 }
 
+// Serial work of a launch in units of one beam tile x one round of 8 time
+// tiles on one CU: rounds per workgroup x beam tiles x workgroups per CU.
+static int64_t launch_cost(const MfmaConfig &c, int num_cus) {
+  const int64_t per_cu = (c.nwork + num_cus - 1) / num_cus;
+  return per_cu * ((c.tiles_per_chunk + kWaves - 1) / kWaves) * c.nt;
+}
+
 // The MFMA configuration of a launch over T times and F frequencies: the
 // plan's, with its grid for (T, F), or, when that leaves CUs without a
-// workgroup, the one-beam-tile configuration if it gives more workgroups
-// (each with 1/NT of the serial MFMA work: a 1024-sample chunk of one
-// channel at D = 512, B = 96 takes 6.4 us instead of 11.9 us back to back).
+// workgroup, the one-beam-tile configuration if its serial work is clearly
+// smaller (each workgroup does 1/NT of the MFMA work, but every beam group
+// reads and unpacks E again: a 1024-sample chunk of one channel at D = 512,
+// B = 96 takes 6.4 us instead of 11.9 us back to back, while a launch that
+// leaves only a few CUs idle keeps the plan's tiling).
 bool pick_config(const bf_plan *p, int64_t T, int64_t F, MfmaConfig *c) {
   *c = p->cfg;
   if (!choose_grid(T, F, (int64_t)p->num_cus * p->wgs_per_cu, c)) return false;
   if (p->has_narrow && c->nwork < p->num_cus) {
     MfmaConfig n = p->cfg_narrow;
-    if (choose_grid(T, F, (int64_t)p->num_cus * p->narrow_wgs_per_cu, &n) && n.nwork > c->nwork) *c = n;
+    if (choose_grid(T, F, (int64_t)p->num_cus * p->narrow_wgs_per_cu, &n) && n.nwork > c->nwork &&
+        5 * launch_cost(n, p->num_cus) < 4 * launch_cost(*c, p->num_cus))
+      *c = n;
   }
   return true;
 }

@@ -32,8 +32,8 @@ struct bf_batch {
   int grid = 0;
   int grid_narrow = 0; // with the plan's one-beam-tile configuration (0: none)
   // Two descriptor blocks in pinned host memory, used by alternate runs and
-  // read by the kernel in place: [work item count | pad to 16][chunks:
-  // max_chunks][work item starts: item_capacity + 1, padded to 16][segments:
+  // read by the kernel in place: [work item count, queue base, mode | pad to
+  // 16][work item starts: item_capacity + 1, padded to 16][segment records:
   // item_capacity]. A run rewrites its block only after the launch that last
   // read it (two runs before) has finished.
   struct Block {
@@ -50,7 +50,7 @@ struct bf_batch {
   } blk[2];
   int next_block = 0;
   unsigned *d_heads = nullptr; // the two queue counters, 256 bytes apart
-  size_t off_chunks = 16, off_work = 0, off_items = 0, desc_bytes = 0;
+  size_t off_work = 16, off_items = 0, desc_bytes = 0;
   hipStream_t capture_stream = nullptr;
   uint64_t plan_version = 0; // the plan's layout_version the grid and the graphs were made for
 };
@@ -94,10 +94,9 @@ int ensure_item_capacity(bf_batch *b, int n) {
   }
   b->item_capacity = 0;
   auto up16 = [](size_t x) { return (x + 15) / 16 * 16; };
-  b->off_chunks = 16;
-  b->off_work = b->off_chunks + up16((size_t)b->max_chunks * sizeof(BatchChunk));
+  b->off_work = 16;
   b->off_items = b->off_work + up16((size_t)(cap + 1) * sizeof(int));
-  b->desc_bytes = b->off_items + (size_t)cap * sizeof(BatchItem);
+  b->desc_bytes = b->off_items + (size_t)cap * sizeof(SegRec);
   for (bf_batch::Block &k : b->blk) {
     // coherent: the device reads what the host wrote before the launch, uncached
     BF_HIP(hipHostMalloc(&k.h, b->desc_bytes, hipHostMallocCoherent));
@@ -199,10 +198,8 @@ int prepare(bf_batch *b, const bf_chunk *chunks, int n, int *block) {
   const int j = b->next_block;
   bf_batch::Block &k = b->blk[j];
   if (k.launch_recorded) BF_HIP(hipEventSynchronize(k.launch_done)); // the run two before still reads it
-  BatchChunk *hc = reinterpret_cast<BatchChunk *>(k.h + b->off_chunks);
-  for (int i = 0; i < n; ++i)
-    hc[i] = BatchChunk{chunks[i].E, chunks[i].J, chunks[i].ntimes, chunks[i].channel, chunk_nfreq(chunks[i])};
-  std::copy(items.begin(), items.end(), reinterpret_cast<BatchItem *>(k.h + b->off_items));
+  SegRec *hs = reinterpret_cast<SegRec *>(k.h + b->off_items);
+  for (size_t i = 0; i < items.size(); ++i) hs[i] = seg_record(chunks[items[i].chunk], items[i], p->shape.ndishes);
   std::copy(work.begin(), work.end(), reinterpret_cast<int *>(k.h + b->off_work));
   reinterpret_cast<int *>(k.h)[0] = nwork;
   reinterpret_cast<uint32_t *>(k.h)[1] = k.base;
@@ -222,8 +219,7 @@ BatchParams batch_params(const bf_batch *b, int j) {
   const bf_plan *p = b->plan;
   const uint8_t *d = b->blk[j].d;
   BatchParams a;
-  a.chunks = reinterpret_cast<const BatchChunk *>(d + b->off_chunks);
-  a.items = reinterpret_cast<const BatchItem *>(d + b->off_items);
+  a.segs = reinterpret_cast<const SegRec *>(d + b->off_items);
   a.work_start = reinterpret_cast<const int *>(d + b->off_work);
   a.nitems = reinterpret_cast<const int *>(d);
   a.head = b->blk[j].head;

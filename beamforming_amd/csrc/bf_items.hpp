@@ -23,6 +23,35 @@ struct BatchItem {
 
 inline int chunk_nfreq(const bf_chunk &k) { return k.nfreq == 0 ? 1 : k.nfreq; }
 
+// A segment as the batch kernel reads it: self-contained, 48 bytes (three
+// 16-byte pieces), so that a workgroup loads all segments of its work item in
+// one round trip.
+struct SegRec {
+  const uint8_t *E;  // E byte of (t = 0, d = 0, p = 0) of the segment's frequency
+  uint8_t *J;        // J of the chunk
+  int64_t T;         // time samples of the chunk
+  int32_t channel;   // channel-table row of the segment's frequency
+  int32_t nf;        // frequencies of the chunk
+  int32_t f_bg;      // frequency within the chunk | beam group << 16
+  int32_t tile_begin, tile_end;
+  int32_t reserved;
+};
+static_assert(sizeof(SegRec) == 48, "three 16-byte pieces");
+
+inline SegRec seg_record(const bf_chunk &k, const BatchItem &s, int64_t D) {
+  SegRec r;
+  r.E = k.E + (int64_t)s.f * D * 2;
+  r.J = k.J;
+  r.T = k.ntimes;
+  r.channel = k.channel + s.f;
+  r.nf = chunk_nfreq(k);
+  r.f_bg = (int32_t)s.f | ((int32_t)s.bg << 16);
+  r.tile_begin = s.tile_begin;
+  r.tile_end = s.tile_end;
+  r.reserved = 0;
+  return r;
+}
+
 // Work items with a fixed size of `it` tiles, in chunk order.
 inline void fixed_items(const bf_chunk *chunks, int n, int nbg, int it, std::vector<BatchItem> &items) {
   for (int i = 0; i < n; ++i) {

@@ -125,15 +125,37 @@ __device__ __forceinline__ void stage_weights(const v4i *W, const int *corr, con
   constexpr int KQ = KS / 2, NQ = 2 * NT;
   const int NQP = 2 * NTP;
   const v4i *src = W + ((size_t)wrow * KQ * NQP + (size_t)bg * NQ) * 64;
-  for (int i = threadIdx.x; i < KQ * NQ * 64; i += kThreads) {
-    const int s = i / (NQ * 64);
-    const int rem = i - s * (NQ * 64);
-    Wl[i] = src[(size_t)s * NQP * 64 + rem];
-  }
+  // Every load is issued before the first LDS write, so the copy costs one
+  // memory latency instead of one per 8 KiB (up to 16 v4i per thread).
+  static_assert(NT * 32 <= kThreads, "one offset and gain per thread");
   const size_t g0 = (size_t)wrow * NTP * 32 + (size_t)bg * NT * 32;
-  for (int i = threadIdx.x; i < NT * 32; i += kThreads) {
-    Cl[i] = make_int2(corr[(g0 + i) * 2 + 0], corr[(g0 + i) * 2 + 1]);
-    Gl[i] = G[g0 + i];
+  const bool cg = threadIdx.x < NT * 32;
+  int2 cv = make_int2(0, 0);
+  float gv = 0.0f;
+  if (cg) {
+    cv = make_int2(corr[(g0 + threadIdx.x) * 2 + 0], corr[(g0 + threadIdx.x) * 2 + 1]);
+    gv = G[g0 + threadIdx.x];
+  }
+  constexpr int N = KQ * NQ * 64;
+  constexpr int PER = (N + kThreads - 1) / kThreads;
+  v4i tmp[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = threadIdx.x + k * kThreads;
+    if (N % kThreads == 0 || i < N) {
+      const int s = i / (NQ * 64);
+      const int rem = i - s * (NQ * 64);
+      tmp[k] = src[(size_t)s * NQP * 64 + rem];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = threadIdx.x + k * kThreads;
+    if (N % kThreads == 0 || i < N) Wl[i] = tmp[k];
+  }
+  if (cg) {
+    Cl[threadIdx.x] = cv;
+    Gl[threadIdx.x] = gv;
   }
 }
 
@@ -576,18 +598,12 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
 // per workgroup, so its items are the counter values base .. base + nitems - 1,
 // with base written by the host beside the item count, and nothing has to
 // reset it. The descriptors (item count, base and mode, item starts,
-// segments, chunks) are read straight from pinned host memory, so a launch
-// needs no copy before it.
-struct BatchChunk {
-  const uint8_t *E; // [T][nf][D][2]
-  uint8_t *J;       // [B][nf][2][T] or [B][nf][T][2]
-  int64_t T;
-  int32_t channel;  // channel-table row of frequency 0 of the chunk
-  int32_t nf;       // frequencies in the chunk
-};
+// self-contained segment records) are read straight from pinned host
+// memory, so a launch needs no copy before it; a workgroup reads its item's
+// bounds, then all of its segment records at once into LDS (two round trips
+// over the bus per item, not two per segment).
 struct BatchParams {
-  const BatchChunk *chunks;
-  const BatchItem *items;   // segments
+  const SegRec *segs;       // segments, work item by work item
   const int *work_start;    // item w = segments work_start[w] .. work_start[w + 1] - 1
   const int *nitems;        // [number of work items, queue base, 1: queue / 0: one item per workgroup]
   unsigned *head;           // device: the queue counter
@@ -598,6 +614,9 @@ struct BatchParams {
   int32_t stage_j; // 1: J rows go through the LDS staging area (allocated in the launch's LDS)
 };
 
+// Segment records a workgroup holds in LDS at a time.
+constexpr int kSegBatch = 64;
+
 template <int NT, int KS, bool DT, int JL>
 __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a) {
   extern __shared__ v4i Wl[];
@@ -605,6 +624,7 @@ __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a
   float *Gl = reinterpret_cast<float *>(Cl + NT * 32);
   v4i *Xl = reinterpret_cast<v4i *>(Gl + NT * 32); // the J staging area, where the launch has one
   __shared__ int s_item;
+  __shared__ SegRec s_seg[kSegBatch];
   v4i *Sl = mfma_stages_j(NT, KS) && a.stage_j ? Xl : nullptr; // compiled out where it cannot fit
   int loaded_channel = -1, loaded_bg = -1;
   const int nitems = a.nitems[0];
@@ -623,30 +643,45 @@ __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a
       break;
     }
     if (it >= nitems) break; // uniform: the same value in every thread
-    const int seg_end = a.work_start[it + 1];
-    for (int sg = a.work_start[it]; sg < seg_end; ++sg) {
-      const BatchItem item = a.items[sg];
-      const BatchChunk ch = a.chunks[item.chunk];
-      const int channel = ch.channel + item.f;
-      if (channel != loaded_channel || item.bg != loaded_bg) {
-        stage_weights<NT, KS>(a.W, a.corr, a.G, a.NTP, channel, item.bg, Wl, Cl, Gl);
-        loaded_channel = channel;
-        loaded_bg = item.bg;
-      }
+    const int s0 = a.work_start[it], s1 = a.work_start[it + 1];
+    for (int sb = s0; sb < s1; sb += kSegBatch) {
+      // the item's segment records, up to kSegBatch at a time, in one
+      // round trip to the descriptor block
+      const int nb = s1 - sb < kSegBatch ? s1 - sb : kSegBatch;
+      if ((int)threadIdx.x < 3 * nb)
+        reinterpret_cast<v4i *>(s_seg)[threadIdx.x] = reinterpret_cast<const v4i *>(a.segs + sb)[threadIdx.x];
       __syncthreads();
-      TileJob job;
-      job.E = ch.E + (int64_t)item.f * a.D * 2;
-      job.e_tstride = (int64_t)ch.nf * a.D * 2;
-      job.J = ch.J;
-      job.T = ch.T;
-      job.F = ch.nf;
-      job.f = item.f;
-      job.D = a.D;
-      job.B = a.B;
-      job.bg = item.bg;
-      job.jlayout = a.jlayout;
-      run_tiles<NT, KS, DT, JL>(job, item.tile_begin, item.tile_end, Wl, Cl, Gl, Sl);
-      __syncthreads(); // the LDS weights and staging are reused by the next segment
+      for (int k = 0; k < nb; ++k) {
+        // wave-uniform copies of the record (scalar control flow below)
+        const SegRec &r = s_seg[k];
+        auto uni = [](int32_t x) { return __builtin_amdgcn_readfirstlane(x); };
+        auto uni64 = [&](int64_t x) {
+          return (int64_t)(((uint64_t)(uint32_t)uni((int32_t)(x >> 32)) << 32) | (uint32_t)uni((int32_t)x));
+        };
+        const int channel = uni(r.channel);
+        const int f_bg = uni(r.f_bg);
+        const int f = f_bg & 0xffff, bg = f_bg >> 16;
+        if (channel != loaded_channel || bg != loaded_bg) {
+          stage_weights<NT, KS>(a.W, a.corr, a.G, a.NTP, channel, bg, Wl, Cl, Gl);
+          loaded_channel = channel;
+          loaded_bg = bg;
+        }
+        __syncthreads();
+        TileJob job;
+        job.E = reinterpret_cast<const uint8_t *>(uni64(reinterpret_cast<int64_t>(r.E)));
+        const int nf = uni(r.nf);
+        job.e_tstride = (int64_t)nf * a.D * 2;
+        job.J = reinterpret_cast<uint8_t *>(uni64(reinterpret_cast<int64_t>(r.J)));
+        job.T = uni64(r.T);
+        job.F = nf;
+        job.f = f;
+        job.D = a.D;
+        job.B = a.B;
+        job.bg = bg;
+        job.jlayout = a.jlayout;
+        run_tiles<NT, KS, DT, JL>(job, uni(r.tile_begin), uni(r.tile_end), Wl, Cl, Gl, Sl);
+        __syncthreads(); // the LDS weights, staging and segment records are reused next
+      }
     }
     if (queue) __syncthreads(); // s_item is rewritten by the next pop
   }

@@ -212,11 +212,16 @@ void check_batch(int id) {
     bufs[i].e_bytes = chunks[i].ntimes * nf * D * 2;
     bufs[i].jcount.assign((size_t)(B * nf * 2 * chunks[i].ntimes), 0);
   }
+  // through the segment records the kernel reads (E addresses as offsets
+  // from a chunk base that is never dereferenced)
   for (const BatchItem &s : segs) {
-    const bf_chunk &k = chunks[s.chunk];
-    const int nf = chunk_nfreq(k);
-    run_job(c, bufs[s.chunk], (int64_t)s.f * D * 2, (int64_t)nf * D * 2, k.ntimes, nf, s.f, D, B, s.bg, s.tile_begin,
-            s.tile_end);
+    bf_chunk k = chunks[s.chunk];
+    k.E = reinterpret_cast<const uint8_t *>(uintptr_t(1) << 40);
+    const SegRec r = seg_record(k, s, D);
+    expect(r.nf == chunk_nfreq(chunks[s.chunk]) && r.T == k.ntimes && r.channel == k.channel + s.f,
+           "segment record fields", s.chunk, s.f);
+    run_job(c, bufs[s.chunk], (int64_t)(r.E - k.E), (int64_t)r.nf * D * 2, r.T, r.nf, r.f_bg & 0xffff, D, B,
+            r.f_bg >> 16, r.tile_begin, r.tile_end);
   }
   for (int i = 0; i < n; ++i) check_written_once(bufs[i], "batched launch: every chunk's J byte written once", id);
 }
