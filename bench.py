@@ -38,7 +38,7 @@ import time
 
 import numpy as np
 
-MI355X_INT8_TOPS = 5000.0  # dense int8 MFMA, 5.0 POP/s
+MI355X_INT8_TOPS = 5000.0  # dense int8 and fp8 MFMA, 5.0 POP/s
 MI355X_HBM_GBPS = 8000.0   # HBM3E, 8 TB/s
 
 
@@ -822,13 +822,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int8",
+        "dtype": "fp8",
         "data": "synthetic (seeded uniform ucomplex4 E and A in [-7,7]^2, reference gain formula)",
         "config": {"workload": "full-size single shot", "ntimes": T, "nfrequencies": F, "ndishes": D,
                    "npolarizations": 2, "nbeams": B, "j_layout": "[b][f][p][t]"},
         "roofline": {"bound": "mfma", "achieved": round(tops, 2), "peak": MI355X_INT8_TOPS, "unit": "TFLOP/s",
                      "frac": round(tops / MI355X_INT8_TOPS, 4), "traffic": traffic,
-                     "note": "int8 MFMA ops per second vs the 5.0 POP/s dense int8 peak; achieved = algorithmic "
+                     "note": "MFMA ops per second (fp8 e4m3 x e5m2 operands holding the int4 values exactly, "
+                             "fp32 accumulation of exact integer sums) vs the 5.0 POP/s dense fp8 (= int8) peak; "
+                             "achieved = algorithmic "
                              "ops per launch / mean launch time (HIP events on the launch stream); traffic = "
                              "HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated"},
         "beam_samples_per_s": B * F * T * 2 / (ms * 1e-3),
