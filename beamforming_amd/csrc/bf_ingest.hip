@@ -83,6 +83,13 @@ __global__ __launch_bounds__(kScatterThreads) void ingest_scatter_kernel(const S
   const int64_t dst_off = (b.dst_row[k] + t) * width;
   const uint32_t wu = (uint32_t)(width / 16);
   const uint32_t units = (uint32_t)ngroups * wu;
+  // The row's destination in every group's slot, looked up once per
+  // workgroup: a store then waits on no memory read for its address.
+  __shared__ __attribute__((address_space(1))) v4u *s_dst[kScatterGroups];
+  for (int i = threadIdx.x; i < ngroups; i += kScatterThreads)
+    s_dst[i] = (__attribute__((address_space(1))) v4u *)(slot_table[(int64_t)(grp0 + i) * ring_slots + pick.slot[i]] +
+                                                          dst_off);
+  __syncthreads();
   for (uint32_t u0 = threadIdx.x; u0 < units; u0 += kScatterThreads * kPieces) {
     v4u v[kPieces];
 #pragma unroll
@@ -95,8 +102,7 @@ __global__ __launch_bounds__(kScatterThreads) void ingest_scatter_kernel(const S
       const uint32_t u = u0 + (uint32_t)(i * kScatterThreads);
       if (u < units) {
         const uint32_t g = u / wu, q = u - g * wu;
-        uint8_t *dst = slot_table[(int64_t)(grp0 + (int)g) * ring_slots + pick.slot[g]] + dst_off + 16 * (int64_t)q;
-        __builtin_nontemporal_store(v[i], reinterpret_cast<v4u *>(dst));
+        __builtin_nontemporal_store(v[i], s_dst[g] + q);
       }
     }
   }
