@@ -392,3 +392,34 @@ def test_batch_unaligned_chunks():
                 J = dJ.cpu().numpy()
                 assert (J[:jo] == 0).all() and (J[jo + B * 2 * T:] == 0).all(), f"chunk {i}: bytes outside J written"
                 assert np.array_equal(J[jo:jo + B * 2 * T], oracle_chunk(E, A, G, c, T, D, B)), f"chunk {i} graph={graph}"
+
+
+def test_batch_items_of_many_segments():
+    # thousands of short chunks: a balanced work item then spans far more
+    # segments than a workgroup holds in LDS at a time (64), so the kernel
+    # reads its segment records in several batches; every chunk stays exact
+    C, D, B = 8, 64, 32
+    A, G = channel_table(C, D, B, 51)
+    rng = np.random.default_rng(52)
+    n = 80000  # one round each: about 80 segments per item on up to 1024 workgroups
+    Ts = rng.integers(1, 25, n)
+    chans = rng.integers(0, C, n)
+    e_off = np.concatenate([[0], np.cumsum(Ts * D * 2)])
+    j_off = np.concatenate([[0], np.cumsum(Ts * B * 2)])
+    E = bo.fill_uniform(int(e_off[-1]), 53)
+    dE = torch.from_numpy(E).cuda()
+    dJ = torch.zeros(int(j_off[-1]), dtype=torch.uint8, device="cuda")
+    chunks = [(dE.data_ptr() + int(e_off[i]), dJ.data_ptr() + int(j_off[i]), int(Ts[i]), int(chans[i]))
+              for i in range(n)]
+    with bf.Plan(bf.Shape.make(1, C, D, B), A, G) as plan, bf.Batch(plan, n) as batch:
+        batch.run(chunks, stream())
+        torch.cuda.synchronize()
+        desc = batch.describe()
+        say(desc)
+        items = int(desc.split(" items=")[1].split()[0])
+        segments = int(desc.split(" segments=")[1].split()[0])
+        assert segments > 64 * items, "few segments per item: the LDS batching is not exercised"
+    J = dJ.cpu().numpy()
+    for i in range(n):
+        want = oracle_chunk(E[e_off[i]:e_off[i + 1]], A, G, int(chans[i]), int(Ts[i]), D, B)
+        assert np.array_equal(J[j_off[i]:j_off[i + 1]], want), f"chunk {i} (T={Ts[i]}, channel={chans[i]})"
