@@ -67,13 +67,18 @@ inline void fixed_items(const bf_chunk *chunks, int n, int nbg, int it, std::vec
 // Balanced work items. A workgroup runs a segment of m tiles in
 // ceil(m / 8) rounds (one tile per wave per round), so the cost of a
 // (chunk, frequency, beam group) piece is counted in rounds, its last one
-// possibly partial. The pieces are laid end to end and cut into one share of
-// ceil(total rounds / grid) rounds per workgroup; a cut inside a piece falls
-// on a round boundary, and a share that crosses a piece boundary becomes
-// several segments of one item. Every item then costs at most the share in
-// rounds, the E pipeline restarts only at segment starts, and the weights
-// are restaged only where a share crosses into another channel. Returns the
-// share in rounds.
+// possibly partial. Each segment after an item's first adds the cost of a
+// segment start (the E pipeline refilled, a barrier, the weights restaged
+// where the channel changes), about half a round. The pieces are laid end
+// to end and cut into items of at most `cap` in cost, the smallest cap (in
+// quarter rounds, from ceil(total rounds / grid) rounds up) that needs at
+// most `grid` items; a cut inside a piece falls on a round boundary, and an
+// item that crosses a piece boundary becomes several segments. Every item
+// then costs at most the cap, the E pipeline restarts only at segment
+// starts, and the weights are restaged only where an item crosses into
+// another channel. Returns the largest number of rounds an item may hold.
+constexpr int64_t kSegmentStartQ = 2; // cost of a segment start, in quarter rounds
+
 inline int64_t balanced_items(const bf_chunk *chunks, int n, int nbg, int grid, std::vector<BatchItem> &segs,
                        std::vector<int> &work) {
   int64_t total_rounds = 0;
@@ -81,28 +86,44 @@ inline int64_t balanced_items(const bf_chunk *chunks, int n, int nbg, int grid, 
     const int64_t nt = (chunks[i].ntimes + kTileT - 1) / kTileT;
     total_rounds += (int64_t)chunk_nfreq(chunks[i]) * nbg * ((nt + kWaves - 1) / kWaves);
   }
-  const int64_t share = std::max<int64_t>(1, (total_rounds + grid - 1) / grid);
-  int64_t fill = 0;
-  work.push_back(0);
-  for (int i = 0; i < n; ++i) {
-    const int64_t nt = (chunks[i].ntimes + kTileT - 1) / kTileT;
-    const int nf = chunk_nfreq(chunks[i]);
-    for (int f = 0; f < nf; ++f)
-      for (int bg = 0; bg < nbg; ++bg)
-        for (int64_t t = 0; t < nt;) {
-          const int64_t rounds = std::min((nt - t + kWaves - 1) / kWaves, share - fill);
-          const int64_t take = std::min(nt - t, rounds * kWaves);
-          segs.push_back(BatchItem{i, (uint16_t)f, (uint16_t)bg, (int32_t)t, (int32_t)(t + take)});
-          t += take;
-          fill += rounds;
-          if (fill == share) {
-            work.push_back((int)segs.size());
-            fill = 0;
+  const size_t segs0 = segs.size(), work0 = work.size();
+  // items of cost at most capq quarter rounds; returns their number
+  auto fill = [&](int64_t capq) -> int64_t {
+    segs.resize(segs0);
+    work.resize(work0);
+    work.push_back((int)segs.size());
+    int64_t cost = 0, nitems = 0;
+    for (int i = 0; i < n; ++i) {
+      const int64_t nt = (chunks[i].ntimes + kTileT - 1) / kTileT;
+      const int nf = chunk_nfreq(chunks[i]);
+      for (int f = 0; f < nf; ++f)
+        for (int bg = 0; bg < nbg; ++bg)
+          for (int64_t t = 0; t < nt;) {
+            const bool open = (int)segs.size() > work.back(); // the item holds a segment
+            const int64_t extra = open ? kSegmentStartQ : 0;
+            const int64_t avail = (capq - cost - extra) / 4; // whole rounds that still fit
+            if (avail <= 0) { // close the item (never empty: a fresh item has room for a round)
+              work.push_back((int)segs.size());
+              ++nitems;
+              cost = 0;
+              continue;
+            }
+            const int64_t rounds = std::min((nt - t + kWaves - 1) / kWaves, avail);
+            const int64_t take = std::min(nt - t, rounds * kWaves);
+            segs.push_back(BatchItem{i, (uint16_t)f, (uint16_t)bg, (int32_t)t, (int32_t)(t + take)});
+            t += take;
+            cost += 4 * rounds + extra;
           }
-        }
-  }
-  if (fill > 0) work.push_back((int)segs.size());
-  return share;
+    }
+    if ((int)segs.size() > work.back()) {
+      work.push_back((int)segs.size());
+      ++nitems;
+    }
+    return nitems;
+  };
+  int64_t capq = 4 * std::max<int64_t>(1, (total_rounds + grid - 1) / grid);
+  while (fill(capq) > grid) capq += std::max<int64_t>(1, capq / 32);
+  return capq / 4;
 }
 
 } // namespace bf
