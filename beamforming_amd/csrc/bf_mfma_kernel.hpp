@@ -643,7 +643,9 @@ __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a
       break;
     }
     if (it >= nitems) break; // uniform: the same value in every thread
-    const int s0 = a.work_start[it], s1 = a.work_start[it + 1];
+    // queue items hold one segment each (segment `it`): no read of the
+    // item bounds over the bus
+    const int s0 = queue ? it : a.work_start[it], s1 = queue ? it + 1 : a.work_start[it + 1];
     for (int sb = s0; sb < s1; sb += kSegBatch) {
       // the item's segment records, up to kSegBatch at a time, in one
       // round trip to the descriptor block
