@@ -299,7 +299,7 @@ def run_batched(bf, torch, stream, steps, warmup, seed=7, C=64, D=512, B=96):
     plan = bf.Plan(bf.Shape.make(1, C, D, B), A, G)
     batch = bf.Batch(plan, C)
     chunks = [(dE.data_ptr() + e_off[c], dJb.data_ptr() + j_off[c], Ts[c], c) for c in range(C)]
-    tune = batch.autotune(chunks, sptr, iters=3)
+    tune = batch.autotune(chunks, sptr, iters=5)
 
     def plain():
         for c in range(C):

@@ -245,8 +245,10 @@ int bf_plan_configure(bf_plan *plan, int beam_tiles, int stage_j, double wg_scal
 int bf_plan_autotune(bf_plan *plan, uint8_t *J, const uint8_t *E, void *stream, int iters, char *report,
                      size_t report_len);
 
-/* Times candidate work-item sizes for the given chunk list (one persistent
- * launch each, `iters` times) and keeps the fastest. Synchronous. */
+/* Times candidate work-item sizes for the given chunk list (balanced items,
+ * or fixed sizes of 16 to 256 tiles; one persistent launch each, `iters`
+ * times, in three interleaved passes of which the first only warms the
+ * board) and keeps the fastest. Synchronous. */
 int bf_batch_autotune(bf_batch *batch, const bf_chunk *chunks, int nchunks, void *stream, int iters, char *report,
                       size_t report_len);
 
