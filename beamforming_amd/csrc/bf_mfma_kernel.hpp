@@ -138,10 +138,17 @@ __device__ __forceinline__ void stage_weights(const v4i *W, const int *corr, con
   }
   constexpr int N = KQ * NQ * 64;
   constexpr int PER = (N + kThreads - 1) / kThreads;
+  // The thread index made opaque, so that the PER source offsets are
+  // computed here: hoisted out of a caller's loop (the batch kernel stages
+  // weights per segment) they were held in registers, spilled, and reloaded
+  // between the loads with a full wait each, which turned one memory latency
+  // into six (10 us instead of 2.5 at D = 512, NT = 3).
+  int tid = threadIdx.x;
+  __asm__ volatile("" : "+v"(tid));
   v4i tmp[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
-    const int i = threadIdx.x + k * kThreads;
+    const int i = tid + k * kThreads;
     if (N % kThreads == 0 || i < N) {
       const int s = i / (NQ * 64);
       const int rem = i - s * (NQ * 64);
@@ -150,7 +157,7 @@ __device__ __forceinline__ void stage_weights(const v4i *W, const int *corr, con
   }
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
-    const int i = threadIdx.x + k * kThreads;
+    const int i = tid + k * kThreads;
     if (N % kThreads == 0 || i < N) Wl[i] = tmp[k];
   }
   if (cg) {
