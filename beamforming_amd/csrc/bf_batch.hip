@@ -33,9 +33,13 @@ struct bf_batch {
   int grid_narrow = 0; // with the plan's one-beam-tile configuration (0: none)
   // Two descriptor blocks in pinned host memory, used by alternate runs and
   // read by the kernel in place: [work item count, queue base, mode | pad to
-  // 16][work item starts: item_capacity + 1, padded to 16][segment records:
-  // item_capacity]. A run rewrites its block only after the launch that last
-  // read it (two runs before) has finished.
+  // 16][per work item, the range of its further segment records: 2 x
+  // item_capacity ints, padded to 16][segment records: item_capacity; item
+  // w's first at index w, then the others of every item]. Both tables hold
+  // at least one entry per workgroup of a launch (each workgroup reads its
+  // own item's entries before it knows the item count). A run rewrites its
+  // block only after the launch that last read it (two runs before) has
+  // finished.
   struct Block {
     uint8_t *h = nullptr; // host address
     uint8_t *d = nullptr; // the same memory as the kernel addresses it
@@ -50,7 +54,7 @@ struct bf_batch {
   } blk[2];
   int next_block = 0;
   unsigned *d_heads = nullptr; // the two queue counters, 256 bytes apart
-  size_t off_work = 16, off_items = 0, desc_bytes = 0;
+  size_t off_extra = 16, off_items = 0, desc_bytes = 0;
   hipStream_t capture_stream = nullptr;
   uint64_t plan_version = 0; // the plan's layout_version the grid and the graphs were made for
 };
@@ -94,8 +98,8 @@ int ensure_item_capacity(bf_batch *b, int n) {
   }
   b->item_capacity = 0;
   auto up16 = [](size_t x) { return (x + 15) / 16 * 16; };
-  b->off_work = 16;
-  b->off_items = b->off_work + up16((size_t)(cap + 1) * sizeof(int));
+  b->off_extra = 16;
+  b->off_items = b->off_extra + up16((size_t)2 * cap * sizeof(int));
   b->desc_bytes = b->off_items + (size_t)cap * sizeof(SegRec);
   for (bf_batch::Block &k : b->blk) {
     // coherent: the device reads what the host wrote before the launch, uncached
@@ -194,13 +198,18 @@ int prepare(bf_batch *b, const bf_chunk *chunks, int n, int *block) {
     it = (int)(balanced_items(chunks, n, cu.nbg, narrow ? b->grid_narrow : b->grid, items, work) * kWaves);
   }
   const int nwork = (int)work.size() - 1;
-  if (int rc = ensure_item_capacity(b, std::max<int>(1, (int)items.size()))) return rc;
+  const int grid = narrow ? b->grid_narrow : b->grid;
+  if (int rc = ensure_item_capacity(b, std::max<int>({1, (int)items.size(), grid}))) return rc;
   const int j = b->next_block;
   bf_batch::Block &k = b->blk[j];
   if (k.launch_recorded) BF_HIP(hipEventSynchronize(k.launch_done)); // the run two before still reads it
+  // item w's first segment record at w, its others after all first ones
+  std::vector<int> order, extra;
+  descriptor_order(work, order, extra);
   SegRec *hs = reinterpret_cast<SegRec *>(k.h + b->off_items);
-  for (size_t i = 0; i < items.size(); ++i) hs[i] = seg_record(chunks[items[i].chunk], items[i], p->shape.ndishes);
-  std::copy(work.begin(), work.end(), reinterpret_cast<int *>(k.h + b->off_work));
+  for (size_t i = 0; i < order.size(); ++i)
+    hs[i] = seg_record(chunks[items[order[i]].chunk], items[order[i]], p->shape.ndishes);
+  std::copy(extra.begin(), extra.end(), reinterpret_cast<int *>(k.h + b->off_extra));
   reinterpret_cast<int *>(k.h)[0] = nwork;
   reinterpret_cast<uint32_t *>(k.h)[1] = k.base;
   reinterpret_cast<int *>(k.h)[2] = b->item_tiles > 0 ? 1 : 0; // balanced items: one per workgroup, no queue
@@ -220,7 +229,7 @@ BatchParams batch_params(const bf_batch *b, int j) {
   const uint8_t *d = b->blk[j].d;
   BatchParams a;
   a.segs = reinterpret_cast<const SegRec *>(d + b->off_items);
-  a.work_start = reinterpret_cast<const int *>(d + b->off_work);
+  a.extra = reinterpret_cast<const int *>(d + b->off_extra);
   a.nitems = reinterpret_cast<const int *>(d);
   a.head = b->blk[j].head;
   a.W = p->W;
@@ -245,9 +254,11 @@ int enqueue_launch(bf_batch *b, int j, hipStream_t st) {
 }
 
 // After a launch on block j was enqueued: its queue counter will have grown
-// by the item count plus one failed fetch per workgroup.
+// by the items fetched from it (those beyond the grid; every workgroup runs
+// its own item first) plus one failed fetch per workgroup.
 int record_launch(bf_batch *b, int j, hipStream_t st) {
-  if (b->last_queue) b->blk[j].base += (uint32_t)(b->last_nitems + run_grid(b, j));
+  const int grid = run_grid(b, j);
+  if (b->last_queue) b->blk[j].base += (uint32_t)(std::max(0, b->last_nitems - grid) + grid);
   BF_HIP(hipEventRecord(b->blk[j].launch_done, st));
   b->blk[j].launch_recorded = true;
   return BF_OK;

@@ -52,6 +52,24 @@ inline SegRec seg_record(const bf_chunk &k, const BatchItem &s, int64_t D) {
   return r;
 }
 
+// Order of the segment records in a batch's descriptor block: item w's
+// first segment at slot w, then the other segments of every item, item by
+// item; extra[2 w] .. extra[2 w + 1] - 1 are the slots of item w's others.
+// order[slot] is the segment (index into the segment list) at that slot.
+// work holds the item starts (item w = segments work[w] .. work[w + 1] - 1).
+inline void descriptor_order(const std::vector<int> &work, std::vector<int> &order, std::vector<int> &extra) {
+  const int nwork = (int)work.size() - 1;
+  order.assign(work.empty() ? 0 : (size_t)work.back(), -1);
+  extra.assign((size_t)2 * std::max(nwork, 0), 0);
+  int next = nwork;
+  for (int w = 0; w < nwork; ++w) {
+    order[w] = work[w];
+    extra[2 * w] = next;
+    for (int s = work[w] + 1; s < work[w + 1]; ++s) order[next++] = s;
+    extra[2 * w + 1] = next;
+  }
+}
+
 // Work items with a fixed size of `it` tiles, in chunk order.
 inline void fixed_items(const bf_chunk *chunks, int n, int nbg, int it, std::vector<BatchItem> &items) {
   for (int i = 0; i < n; ++i) {

@@ -598,20 +598,22 @@ __global__ __launch_bounds__(kThreads) void form_beams_mfma(const MfmaParams a) 
 // T) in one persistent launch. The host cuts the work into segments (chunk,
 // frequency of the chunk, beam group, tile range) and groups consecutive
 // segments into work items and runs an item's segments in order, restaging
-// the weights in LDS only when the (channel, beam group) changes. Balanced
-// items (at most one per workgroup) go to workgroup `item`; fixed-size items
-// come from a device-side queue. The queue is a counter in device memory
-// that only grows: a launch adds exactly its item count plus one failed fetch
-// per workgroup, so its items are the counter values base .. base + nitems - 1,
-// with base written by the host beside the item count, and nothing has to
-// reset it. The descriptors (item count, base and mode, item starts,
-// self-contained segment records) are read straight from pinned host
-// memory, so a launch needs no copy before it; a workgroup reads its item's
-// bounds, then all of its segment records at once into LDS (two round trips
-// over the bus per item, not two per segment).
+// the weights in LDS only when the (channel, beam group) changes. Workgroup
+// w runs item w first; balanced items (at most one per workgroup) end
+// there, fixed-size items beyond the grid come from a device-side queue.
+// The queue is a counter in device memory that only grows: a launch adds
+// exactly its dynamically fetched items (nitems - grid, if positive) plus
+// one failed fetch per workgroup, so its fetches are the counter values
+// base .. and item grid + (value - base), with base written by the host
+// beside the item count, and nothing has to reset it. The descriptors (item
+// count, base and mode; per item its first segment record at index item
+// and the range of its further records) are read straight from pinned host
+// memory, so a launch needs no copy before it: a workgroup reads the header,
+// its item's first segment record and the range of the others in one round
+// trip over the bus (the others, if any, in one more).
 struct BatchParams {
-  const SegRec *segs;       // segments, work item by work item
-  const int *work_start;    // item w = segments work_start[w] .. work_start[w + 1] - 1
+  const SegRec *segs;       // segment records: item w's first at w, then the others of every item
+  const int *extra;         // item w's further records: segs[extra[2 w] .. extra[2 w + 1] - 1]
   const int *nitems;        // [number of work items, queue base, 1: queue / 0: one item per workgroup]
   unsigned *head;           // device: the queue counter
   const v4i *W;       // channel table [C][KS / 2][2 NTP][64]
@@ -631,35 +633,66 @@ __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a
   float *Gl = reinterpret_cast<float *>(Cl + NT * 32);
   v4i *Xl = reinterpret_cast<v4i *>(Gl + NT * 32); // the J staging area, where the launch has one
   __shared__ int s_item;
+  __shared__ int s_hdr[3];   // item count, queue base, mode
+  __shared__ int s_range[2]; // the item's further segment records
   __shared__ SegRec s_seg[kSegBatch];
   v4i *Sl = mfma_stages_j(NT, KS) && a.stage_j ? Xl : nullptr; // compiled out where it cannot fit
+  const int tx = threadIdx.x;
   int loaded_channel = -1, loaded_bg = -1;
-  const int nitems = a.nitems[0];
-  const unsigned base = (unsigned)a.nitems[1];
-  const bool queue = a.nitems[2] != 0; // uniform over the launch
+  int nitems = 0;
+  unsigned base = 0;
+  bool queue = false;
   for (int round = 0;; ++round) {
-    int it = blockIdx.x;
-    if (queue) {
-      if (threadIdx.x == 0) {
-        const unsigned u = atomicAdd(a.head, 1u) - base; // this launch's item index
-        s_item = u < (unsigned)nitems ? (int)u : nitems;
+    int it = blockIdx.x; // the workgroup's own item first (the host sizes the blocks for every workgroup)
+    if (round > 0) {
+      if (!queue) break;
+      if (tx == 0) {
+        const unsigned u = atomicAdd(a.head, 1u) - base; // this launch's u-th fetch
+        const unsigned left = nitems > (int)gridDim.x ? (unsigned)(nitems - (int)gridDim.x) : 0u;
+        s_item = u < left ? (int)gridDim.x + (int)u : nitems;
       }
       __syncthreads();
       it = __builtin_amdgcn_readfirstlane(s_item);
-    } else if (round > 0) {
-      break;
+      if (it >= nitems) break; // uniform
     }
-    if (it >= nitems) break; // uniform: the same value in every thread
-    // queue items hold one segment each (segment `it`): no read of the
-    // item bounds over the bus
-    const int s0 = queue ? it : a.work_start[it], s1 = queue ? it + 1 : a.work_start[it + 1];
-    for (int sb = s0; sb < s1; sb += kSegBatch) {
-      // the item's segment records, up to kSegBatch at a time, in one
-      // round trip to the descriptor block
-      const int nb = s1 - sb < kSegBatch ? s1 - sb : kSegBatch;
-      if ((int)threadIdx.x < 3 * nb)
-        reinterpret_cast<v4i *>(s_seg)[threadIdx.x] = reinterpret_cast<const v4i *>(a.segs + sb)[threadIdx.x];
-      __syncthreads();
+    // one round trip: the item's first segment record, the range of its
+    // others and, in the first round, the header
+    if (tx < 3) reinterpret_cast<v4i *>(s_seg)[tx] = reinterpret_cast<const v4i *>(a.segs + it)[tx];
+    if (tx == 64) {
+      s_range[0] = a.extra[2 * it];
+      s_range[1] = a.extra[2 * it + 1];
+    }
+    if (round == 0 && tx == 128) {
+      s_hdr[0] = a.nitems[0];
+      s_hdr[1] = a.nitems[1];
+      s_hdr[2] = a.nitems[2];
+    }
+    __syncthreads();
+    if (round == 0) {
+      nitems = __builtin_amdgcn_readfirstlane(s_hdr[0]);
+      base = (unsigned)__builtin_amdgcn_readfirstlane(s_hdr[1]);
+      queue = __builtin_amdgcn_readfirstlane(s_hdr[2]) != 0; // uniform over the launch
+      if (it >= nitems) { // no item of its own: the queue's, if any
+        if (queue) continue;
+        break;
+      }
+    }
+    const int x1 = __builtin_amdgcn_readfirstlane(s_range[1]);
+    int next = __builtin_amdgcn_readfirstlane(s_range[0]);
+    int nb = 1; // records in s_seg
+    for (int batch = 0;; ++batch) {
+      // the further records, up to kSegBatch at a time (after the first
+      // record in the first batch), in one more round trip
+      const int at = batch == 0 ? 1 : 0;
+      const int ne = x1 - next < kSegBatch - at ? x1 - next : kSegBatch - at;
+      if (ne > 0) {
+        if (tx < 3 * ne) reinterpret_cast<v4i *>(s_seg + at)[tx] = reinterpret_cast<const v4i *>(a.segs + next)[tx];
+        next += ne;
+        nb = at + ne;
+        __syncthreads();
+      } else if (batch > 0) {
+        break;
+      }
       for (int k = 0; k < nb; ++k) {
         // wave-uniform copies of the record (scalar control flow below)
         const SegRec &r = s_seg[k];
@@ -691,6 +724,7 @@ __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a
         run_tiles<NT, KS, DT, JL>(job, uni(r.tile_begin), uni(r.tile_end), Wl, Cl, Gl, Sl);
         __syncthreads(); // the LDS weights, staging and segment records are reused next
       }
+      if (next >= x1) break;
     }
     if (queue) __syncthreads(); // s_item is rewritten by the next pop
   }

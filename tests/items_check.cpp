@@ -2,7 +2,9 @@
 // bf_items.hpp) on random chunk lists: every (chunk, frequency, beam group,
 // time tile) is covered by exactly one segment; balanced items number at
 // most `grid`, cost at most the share in rounds, and cut pieces only on
-// round boundaries; fixed items cover tile ranges of at most their size.
+// round boundaries; fixed items cover tile ranges of at most their size;
+// the descriptor block's record order (descriptor_order) puts every
+// segment at one slot, each item's first at the item's index.
 // Prints "ok" and exits 0 when every check passes.
 #include "bf_items.hpp"
 
@@ -53,11 +55,36 @@ void check_cover(const std::vector<bf_chunk> &chunks, int nbg, const std::vector
       }
 }
 
+// The descriptor block's record order: every segment at exactly one slot;
+// item w = its first segment at slot w, then slots extra[2 w] ..
+// extra[2 w + 1] - 1, in the item's order; the others' slots follow the
+// first ones, item by item.
+void check_order(const std::vector<int> &work) {
+  std::vector<int> order, extra;
+  descriptor_order(work, order, extra);
+  const int nwork = (int)work.size() - 1, nseg = work.back();
+  expect((int)order.size() == nseg && (int)extra.size() == 2 * nwork, "table sizes", (long long)order.size(), nseg);
+  std::vector<int> seen((size_t)nseg, 0);
+  for (int v : order)
+    if (v >= 0 && v < nseg) ++seen[(size_t)v];
+  for (int c : seen) expect(c == 1, "each segment at one slot", c);
+  int expect_next = nwork;
+  for (int w = 0; w < nwork; ++w) {
+    expect(order[(size_t)w] == work[w], "first segment at the item's slot", w, order[(size_t)w]);
+    expect(extra[2 * w] == expect_next && extra[2 * w + 1] - extra[2 * w] == work[w + 1] - work[w] - 1,
+           "further segments' slots", w, extra[2 * w]);
+    for (int i = 0; i < work[w + 1] - work[w] - 1 && extra[2 * w] + i < nseg; ++i)
+      expect(order[(size_t)(extra[2 * w] + i)] == work[w] + 1 + i, "further segments in order", w, i);
+    expect_next = extra[2 * w + 1];
+  }
+}
+
 void check_balanced(const std::vector<bf_chunk> &chunks, int nbg, int grid) {
   std::vector<BatchItem> segs;
   std::vector<int> work;
   const int64_t share = balanced_items(chunks.data(), (int)chunks.size(), nbg, grid, segs, work);
   check_cover(chunks, nbg, segs);
+  check_order(work);
   const int nitems = (int)work.size() - 1;
   expect(nitems <= grid, "items <= grid", nitems, grid);
   expect(work.front() == 0 && work.back() == (int)segs.size(), "item starts span the segments");
