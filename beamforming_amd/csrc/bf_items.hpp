@@ -87,7 +87,7 @@ inline void fixed_items(const bf_chunk *chunks, int n, int nbg, int it, std::vec
 // (chunk, frequency, beam group) piece is counted in rounds, its last one
 // possibly partial. Each segment after an item's first adds the cost of a
 // segment start (the E pipeline refilled, a barrier, the weights restaged
-// where the channel changes), about half a round. The pieces are laid end
+// where the channel changes), about a round. The pieces are laid end
 // to end and cut into items of at most `cap` in cost, the smallest cap (in
 // quarter rounds, from ceil(total rounds / grid) rounds up) that needs at
 // most `grid` items; a cut inside a piece falls on a round boundary, and an
@@ -95,7 +95,7 @@ inline void fixed_items(const bf_chunk *chunks, int n, int nbg, int it, std::vec
 // then costs at most the cap, the E pipeline restarts only at segment
 // starts, and the weights are restaged only where an item crosses into
 // another channel. Returns the largest number of rounds an item may hold.
-constexpr int64_t kSegmentStartQ = 2; // cost of a segment start, in quarter rounds
+constexpr int64_t kSegmentStartQ = 4; // cost of a segment start, in quarter rounds
 
 inline int64_t balanced_items(const bf_chunk *chunks, int n, int nbg, int grid, std::vector<BatchItem> &segs,
                        std::vector<int> &work) {

@@ -642,8 +642,13 @@ __global__ __launch_bounds__(kThreads) void form_beams_batch(const BatchParams a
   int nitems = 0;
   unsigned base = 0;
   bool queue = false;
+  // The workgroup's own item first (the host sizes the blocks for every
+  // workgroup): consecutive items (pieces of one chunk, the same weights)
+  // on one XCD, where they share its L2, when the grid spreads evenly over
+  // the 8 XCDs.
+  const int own_item = (gridDim.x & 7) == 0 ? work_of_block(blockIdx.x, gridDim.x >> 3) : blockIdx.x;
   for (int round = 0;; ++round) {
-    int it = blockIdx.x; // the workgroup's own item first (the host sizes the blocks for every workgroup)
+    int it = own_item;
     if (round > 0) {
       if (!queue) break;
       if (tx == 0) {
