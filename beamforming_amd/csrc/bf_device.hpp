@@ -79,51 +79,132 @@ BF_HD v8i unpack_e_pol(const v4i e, uint32_t sel) {
 BF_HD v4i op_v1(const v8i u) { return __builtin_shufflevector(u, u, 2, 3, 4, 5); }
 BF_HD v4i op_v2(const v8i u) { return __builtin_shufflevector(u, u, 0, 1, 2, 3); }
 
-// fp8 form of the same arithmetic (v_mfma_scale_f32_16x16x128_f8f6f4, A in
-// E4M3, B in E5M2, fp32 accumulators). The E operand bytes are the nibble
-// values n = 0..15 above, unchanged: as E4M3 bit patterns (exponent bias 7,
-// 3 mantissa bits) 0..7 are the subnormals n 2^-9 and 8..15 the normals
-// (1 + (n - 8) / 8) 2^-6 = n 2^-9, so the byte n is the number n 2^-9 exactly.
-// A weight w in -8..7 is stored as the E5M2 code of w 2^9 (all of them have
-// at most 3 significant bits, so every one is exact). Each product is then
-// the integer n w, every partial sum an integer of magnitude below 2^17, and
-// the fp32 accumulation is exact: the accumulators end at the same raw sums
-// as the int8 MFMA, as floats. Used for K steps of 64 dishes, KS = 4 .. 32
-// (at KS = 64 the longer unrolled loop does not fit the registers).
-BF_HD constexpr bool mfma_fp8(int ks) { return ks >= 4 && ks <= 32; }
+// K steps of 64 dishes on v_mfma_scale_f32_16x16x128_f8f6f4 with fp6 operands
+// (below) for KS = 4 .. 32; the int8 MFMA for KS = 2 and 64 (at KS = 64 the
+// longer unrolled loop of the 16x16x128 form does not fit the registers).
+BF_HD constexpr bool mfma_f6(int ks) { return ks >= 4 && ks <= 32; }
 // The MFMA instruction of a K-step count, as plan and batch descriptions name it.
 inline const char *mfma_name(int ks) {
-  return mfma_fp8(ks) ? "mfma_f32_16x16x128_f8f6f4(e4m3 x e5m2)" : "mfma_i32_16x16x64_i8";
-}
-BF_HD uint8_t e5m2_times512(int w) {
-  // |w| 2^9 = m 2^e with m in {1, 1.25, 1.5, 1.75}: exponent field e + 15,
-  // mantissa field 4 (m - 1)
-  const int a = w < 0 ? -w : w;
-  if (a == 0) return 0;
-  int e = 9, m4 = 4 * a; // a 2^9 = (m4 / 4) 2^e, normalised below to m4 in 4..7
-  while (m4 >= 8) {
-    m4 >>= 1;
-    ++e;
-  }
-  return (uint8_t)((w < 0 ? 0x80 : 0) | ((e + 15) << 2) | (m4 - 4));
+  return mfma_f6(ks) ? "mfma_f32_16x16x128_f8f6f4(e2m3 x e3m2)" : "mfma_i32_16x16x64_i8";
 }
 
 // Weight operand for one lane: 8 packed A bytes (dishes d0..d0+7, zero-padded
-// by the caller) -> [Ar d0-3 | Ar d4-7 | Ai d0-3 | Ai d4-7] as int8, or with
-// fp8 as the E5M2 codes of 512 times those values.
-BF_HD v4i pack_weights(const uint8_t a[8], bool fp8 = false) {
+// by the caller) -> [Ar d0-3 | Ar d4-7 | Ai d0-3 | Ai d4-7] as int8.
+BF_HD v4i pack_weights(const uint8_t a[8]) {
   uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int q = 0; q < 2; ++q)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const uint32_t x = a[4 * q + i];
-      const uint32_t re = fp8 ? e5m2_times512(dec_re(x)) : (uint32_t)dec_re(x) & 0xffu;
-      const uint32_t im = fp8 ? e5m2_times512(dec_im(x)) : (uint32_t)dec_im(x) & 0xffu;
-      w[q + 0] |= re << (8 * i);
-      w[q + 2] |= im << (8 * i);
+      w[q + 0] |= ((uint32_t)dec_re(x) & 0xffu) << (8 * i);
+      w[q + 2] |= ((uint32_t)dec_im(x) & 0xffu) << (8 * i);
     }
   return v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+}
+
+// ---------------------------------------------------------------------------
+// fp6 form (v_mfma_scale_f32_16x16x128_f8f6f4 with A in E2M3, B in E3M2).
+// In a register-only loop on all CUs (random operands in these ranges) the
+// instruction runs 7.57 POP/s at 1341 W with fp6 operands against 4.82 POP/s
+// at 1285 W with fp8 operands: two thirds of the energy per operation, and
+// the launch runs at the board's power limit. A lane's operand is 32 six-bit
+// fields, field i at bits 6 i of a 6-dword tuple (k = 32 (lane / 16) + i):
+// 16 dishes x (re, im) of a 64-dish step (dishes 8 kg .. 8 kg + 7 of each of
+// its two 32-dish halves).
+//  * E: a nibble n = 0..15 in the low 4 bits of a field (top 2 bits zero) is
+//    the E2M3 code of n / 8 (0..7 subnormal, 8..15 exponent 1), exactly; the
+//    MFMA's A scale 2^3 makes every product the integer n w.
+//  * Weights: w = -8..7 as E3M2 codes (sign, 3 exponent bits of bias 3, 2
+//    mantissa bits: every such w is exact).
+// Every partial sum is an integer below 2^17: the fp32 accumulators end at
+// the same raw sums as the int8 form's int32 accumulators, exactly.
+//
+// The operand halves. For one polarization the 16 dishes' nibbles fill two
+// 16-field halves H1 (dwords 0..2) and H2 (dwords 3..5): field j of H1 holds
+// one nibble of dish kF6Dish[j] and field j of H2 the other. Where H1 holds
+// s_im ("case A") the weight halves are (Ar, Ai), where it holds s_re ("case
+// B") they are (Ai, Ar); then
+//   [H1, H2] . W                     = raw_im + 8 sum Ar + 8 sum Ai
+//   [H2 ^ MR, H1 ^ MI] . W           = raw_re + 8 sum Ar + 7 sum Ai
+// where the masks complement s_im (to 15 - s_im) in the case-B fields of H2
+// (MR) and the case-A fields of H1 (MI): the same nibble-offset corrections
+// as the other forms. The dish of field j (0 .. 7: dishes 8 kg + 0 .. 7 of
+// the step's first 32 dishes, 8 .. 15: of its second 32) and its case:
+BF_HD constexpr int f6_dish(int j) {
+  constexpr int d[16] = {0, 3, 1, 4, 2, 5, 8, 6, 9, 7, 10, 13, 11, 14, 12, 15};
+  return d[j];
+}
+BF_HD constexpr bool f6_case_b(int j) { return (0x64CCu >> j) & 1u; } // fields 2 3 6 7 10 13 14
+// E3M2 code of an integer w in -8..8.
+BF_HD uint32_t e3m2_code(int w) {
+  const int a = w < 0 ? -w : w;
+  int c = 0;
+  if (a == 1) c = 12;       // 1.00 x 2^0
+  else if (a == 2) c = 16;  // 1.00 x 2^1
+  else if (a == 3) c = 18;  // 1.50 x 2^1
+  else if (a >= 4) c = 20 + (a - 4); // 4..7: 1.00 .. 1.75 x 2^2; 8: 1.00 x 2^3
+  return (uint32_t)(c | (w < 0 ? 32 : 0));
+}
+// Weight operand halves of one lane and 64-dish step (a0: the 8 dishes of the
+// step's first 32, a1: of its second 32, ucomplex4 bytes, 0x88 where zero):
+// WA (dwords 0..2) and WB (dwords 3..5), field j at bits 6 j.
+BF_HD void pack_weights_f6(const uint8_t a0[8], const uint8_t a1[8], uint32_t w[6]) {
+  for (int i = 0; i < 6; ++i) w[i] = 0;
+  for (int j = 0; j < 16; ++j) {
+    const int dd = f6_dish(j);
+    const uint32_t x = dd < 8 ? a0[dd] : a1[dd - 8];
+    const int ar = dec_re(x), ai = dec_im(x);
+    const uint32_t ca = e3m2_code(f6_case_b(j) ? ai : ar), cb = e3m2_code(f6_case_b(j) ? ar : ai);
+    const int bit = 6 * j, wd = bit >> 5, o = bit & 31;
+    w[wd] |= ca << o;
+    w[3 + wd] |= cb << o;
+    if (o > 26) {
+      w[wd + 1] |= ca >> (32 - o);
+      w[3 + wd + 1] |= cb >> (32 - o);
+    }
+  }
+}
+// E operands of one polarization (sel_add = 0 or 1) of a 64-dish step from
+// the lane's two 16-byte loads (r0..r3 = e0: dishes 8 kg + 0 .. 7 of the
+// first 32, both polarizations, two dishes per dword; r4..r7 = e1). Each
+// 32-bit piece of a half is two byte permutes of the raw dwords (the
+// polarization's bytes, a dish placed twice where the two halves need its
+// two nibbles at different shifts) merged under field masks:
+//   H1: d0 = Pa & Ma | Pb << 6 & Mb,  d1 = Aq & MA | Bq >> 2 & MB,  d2 = Cq & MC | Dq << 2 & MD
+//   H2: d0 = Pa >> 4 & Ma | Pb << 2 & Mb, d1 = Aq << 4 & MA | Bq >> 6 & MB, d2 = Cq << 4 & MC | Dq >> 2 & MD
+// (field 5 straddles d0 and d1 of each half). v2 = [H1, H2], v1 = [H2 ^ MR,
+// H1 ^ MI].
+typedef int v6i __attribute__((ext_vector_type(6)));
+// a | (b & m) in one v_bitop3_b32 (truth table 0xF8: inputs a, b, m as bits
+// 2, 1, 0 of the row index); the compiler otherwise spends two ANDs and an OR.
+BF_HD uint32_t or_masked(uint32_t a, uint32_t b, uint32_t m) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_bitop3_b32(a, b, m, 0xF8);
+#else
+  return a | (b & m);
+#endif
+}
+BF_HD void pack_e_f6(const v4i e0, const v4i e1, uint32_t sel_add, v6i &v2, v6i &v1) {
+  const uint32_t r0 = (uint32_t)e0.x, r1 = (uint32_t)e0.y, r2 = (uint32_t)e0.z, r3 = (uint32_t)e0.w;
+  const uint32_t r4 = (uint32_t)e1.x, r5 = (uint32_t)e1.y, r6 = (uint32_t)e1.z, r7 = (uint32_t)e1.w;
+  // byte selectors for polarization 0; polarization 1 is one byte up
+  const uint32_t sa = 0x04020200u + sel_add, sb = 0x06040402u + sel_add, sc = 0x04040202u + sel_add;
+  const uint32_t pa = perm_bytes(r1, r0, sa); // dishes 0 1 1 2
+  const uint32_t pb = perm_bytes(r2, r1, sb); // dishes 3 4 4 5
+  const uint32_t bq = perm_bytes(r3, r2, sb); // dishes 5 6 6 7
+  const uint32_t aq = perm_bytes(r5, r4, sa); // dishes 8 9 9 10
+  const uint32_t cq = perm_bytes(r6, r5, sc); // dishes 11 11 12 12
+  const uint32_t dq = perm_bytes(r7, r6, sb); // dishes 13 14 14 15
+  const uint32_t h10 = or_masked(pa & 0x0F00F00Fu, pb << 6, 0xC03C03C0u);
+  const uint32_t h11 = or_masked(aq & 0xF00F00F0u, bq >> 2, 0x03C03C03u);
+  const uint32_t h12 = or_masked(cq & 0x00F00F00u, dq << 2, 0x3C03C03Cu);
+  const uint32_t h20 = or_masked((pa >> 4) & 0x0F00F00Fu, pb << 2, 0xC03C03C0u);
+  const uint32_t h21 = or_masked((aq << 4) & 0xF00F00F0u, bq >> 6, 0x03C03C03u);
+  const uint32_t h22 = or_masked((cq << 4) & 0x00F00F00u, dq >> 2, 0x3C03C03Cu);
+  v2 = v6i{(int)h10, (int)h11, (int)h12, (int)h20, (int)h21, (int)h22};
+  v1 = v6i{(int)(h20 ^ 0x003CF000u), (int)(h21 ^ 0xF0003CF0u), (int)(h22 ^ 0x00F3C000u),
+           (int)(h10 ^ 0xCF0003CFu), (int)(h11 ^ 0x03CF0003u), (int)(h12 ^ 0x3C000F3Cu)};
 }
 
 // ---------------------------------------------------------------------------

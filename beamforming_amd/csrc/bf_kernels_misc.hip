@@ -5,12 +5,16 @@
 namespace bf {
 
 // ---------------------------------------------------------------------------
-// Weight operands W[f][s][g][lane] (16 bytes each) for the 16x16x64 MFMA:
-// step s covers dishes 32 s .. 32 s + 31, g is a 16-beam group (2 NTP of
-// them over the NTP 32-beam tiles), lane = (kg, c) = (lane / 16, lane % 16)
-// holds beam 16 g + c, dishes 32 s + 8 kg .. +7 (Ar, then Ai), zero outside
-// B and D; int8, or E5M2 codes of 512 times the values where the kernel runs
-// the fp8 MFMA (KS >= 4). KS counts 16-dish steps (even).
+// Weight operands W[f][s][g][lane] (16 bytes each). g is a 16-beam group (2
+// NTP of them over the NTP 32-beam tiles) and lane = (kg, c) = (lane / 16,
+// lane % 16) serves beam 16 g + c; zero outside B and D. KS counts 16-dish
+// steps (even); s runs over the KS / 2 steps of 32 dishes.
+//  * int8 (16x16x64 MFMA): entry s holds dishes 32 s + 8 kg .. +7, Ar then
+//    Ai (pack_weights).
+//  * fp6 (16x16x128 MFMA, KS = 4 .. 32): entries 2 h and 2 h + 1 hold the
+//    6-dword operand of the 64-dish step h, over dishes 64 h + 8 kg .. +7
+//    and 64 h + 32 + 8 kg .. +7: dwords 0..3 in entry 2 h, 4..5 in entry
+//    2 h + 1 (pack_weights_f6).
 __global__ void prep_weights_kernel(const uint8_t *__restrict__ A, v4i *__restrict__ W, int F, int B, int D, int KS,
                                     int NTP) {
   const int KQ = KS / 2, NQP = 2 * NTP;
@@ -23,14 +27,25 @@ __global__ void prep_weights_kernel(const uint8_t *__restrict__ A, v4i *__restri
     const int s = (int)(q % KQ);
     const int f = (int)(q / KQ);
     const int b = g * 16 + (lane & 15);
-    const int d0 = 2 * kDishesPerStep * s + 8 * (lane >> 4);
-    uint8_t a[8];
+    auto gather = [&](int d0, uint8_t a[8]) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int d = d0 + k;
-      a[k] = (b < B && d < D) ? A[((int64_t)f * B + b) * D + d] : (uint8_t)0x88; // 0x88 decodes to (0, 0)
+      for (int k = 0; k < 8; ++k) {
+        const int d = d0 + k;
+        a[k] = (b < B && d < D) ? A[((int64_t)f * B + b) * D + d] : (uint8_t)0x88; // 0x88 decodes to (0, 0)
+      }
+    };
+    uint8_t a0[8], a1[8];
+    if (mfma_f6(KS)) {
+      const int h = s >> 1;
+      gather(2 * kDishesPerStep * (2 * h) + 8 * (lane >> 4), a0);
+      gather(2 * kDishesPerStep * (2 * h + 1) + 8 * (lane >> 4), a1);
+      uint32_t w[6];
+      pack_weights_f6(a0, a1, w);
+      W[i] = (s & 1) ? v4i{(int)w[4], (int)w[5], 0, 0} : v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+    } else {
+      gather(2 * kDishesPerStep * s + 8 * (lane >> 4), a0);
+      W[i] = pack_weights(a0);
     }
-    W[i] = pack_weights(a, mfma_fp8(KS));
   }
 }
 

@@ -2,11 +2,11 @@
 //
 // GEMM per frequency: rows = time samples of one polarization (16 per MFMA
 // block), columns = beams (16 per MFMA block), K = dishes x (re, im). The
-// MFMA is v_mfma_scale_f32_16x16x128_f8f6f4 (64 dishes per instruction, fp8
+// MFMA is v_mfma_scale_f32_16x16x128_f8f6f4 (64 dishes per instruction, fp6
 // operands holding the int4 values exactly, fp32 accumulators holding exact
 // integer sums) for 64 .. 512 dishes, and v_mfma_i32_16x16x64_i8 (32 dishes
-// per instruction) for fewer or more: at the board's power limit, where the
-// kernel runs, these small shapes sustain the most operations per joule.
+// per instruction) for fewer or more: the launch runs at the board's power
+// limit, where the fp6 form spends the least energy per operation.
 //
 // One workgroup = one (frequency, beam group of NT x 32 beams, time chunk):
 //  * the prepared weight operands of the beam group (KS x NT KiB) are staged
@@ -16,11 +16,10 @@
 //    into a register ring running RP steps ahead, each lane (kg, col) =
 //    (lane / 16, lane % 16) loading the 16 bytes of its own operand: row
 //    (time) col, dishes 8 kg .. 8 kg + 7 (the four lanes of a row read its
-//    64-byte segment); it keeps each polarization with one byte permute per
-//    4 dishes and splits the nibbles into the two operand variants (see
-//    bf_device.hpp);
-//  * per K step (fp8: pair of steps) and 16-beam group: one LDS read of the
-//    weight operand (fp8: two), shared by both polarizations, and per
+//    64-byte segment); byte permutes and masked shifts turn the nibbles of
+//    each polarization into the two operand variants (see bf_device.hpp);
+//  * per K step (fp6: pair of steps) and 16-beam group: one LDS read of the
+//    weight operand (fp6: two), shared by both polarizations, and per
 //    polarization two MFMAs (real and imaginary accumulator);
 //  * the accumulators (int32, or fp32 holding integers exactly) start at
 //    minus the per-beam nibble offsets, so
@@ -211,24 +210,22 @@ __device__ __forceinline__ void store_row4(uint8_t *dst, uint32_t w, int n, bool
 // (kg, col) = (lane / 16, lane % 16) holds, for the E operand, row (time)
 // col and dishes 8 kg .. 8 kg + 7 of the step; for the weight operand, beam
 // col of the group and the same dishes; its accumulators hold rows
-// 4 kg .. 4 kg + 3 (times) of column col (beam). fp8 (KS = 4 .. 32),
-// v_mfma_scale_f32_16x16x128_f8f6f4 with unit scales: the same operands of
-// two consecutive steps side by side (E bytes read as E4M3, weights as
-// E5M2 codes of 512 w; exact, see mfma_fp8), one MFMA per pair of steps and
-// fp32 accumulators. At the board's power limit the smaller shapes sustain
-// more operations per joule: i8 16x16x64 4.7 POP/s against 4.1 for
-// 32x32x32 in a register-only loop, and the fp8 form draws 6% less power
-// than i8 16x16x64 for 3% more operations (1207 W at 4.89 POP/s against
-// 1283 W at 4.73, same loop); the fp8 form also needs no int-to-float
-// conversion in the epilogue.
+// 4 kg .. 4 kg + 3 (times) of column col (beam). fp6 (KS = 4 .. 32),
+// v_mfma_scale_f32_16x16x128_f8f6f4 with E2M3 x E3M2 operands: the dishes
+// of two consecutive steps in one MFMA (see pack_e_f6 and pack_weights_f6),
+// fp32 accumulators of exact integer sums, and no int-to-float conversion in
+// the epilogue. The launch runs at the board's power limit, so energy per
+// operation sets its time: the fp6 form took the full-size launch from
+// 314 to 286 us against the same instruction with fp8 operands (E4M3 x
+// E5M2), which had taken it from 343 us with v_mfma_i32_16x16x64_i8.
 template <int NT, int KS, bool DT, int JL>
 __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile_first, const int64_t tile_end,
                                           const v4i *Wl, const int2 *Cl, const float *Gl, v4i *Sl) {
   static_assert(NT >= 1 && NT <= kMaxBeamTiles, "32-beam tiles per workgroup");
   static_assert(KS >= 2 && KS <= 64 && KS % 2 == 0, "K steps of 16 dishes, in pairs");
   constexpr int KQ = KS / 2;                 // K steps of 32 dishes (one E load each)
-  constexpr bool F8 = mfma_fp8(KS);          // fp8 MFMA over pairs of steps (64 dishes)
-  typedef typename std::conditional<F8, v4f, v4i>::type Acc;
+  constexpr bool F6 = mfma_f6(KS);           // fp6 16x16x128 MFMA over pairs of steps (64 dishes), fp32 accumulators
+  typedef typename std::conditional<F6, v4f, v4i>::type Acc;
   constexpr int NQ = 2 * NT;                 // 16-beam column groups
   constexpr int RP0 = NT >= 3 ? 4 : 8;       // E prefetch distance in steps (fewer registers at NT >= 3)
   constexpr int RP = RP0 < KQ ? RP0 : KQ;
@@ -288,7 +285,7 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
       const int2 c = Cl[qq * 16 + col];
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
-        if constexpr (F8) {
+        if constexpr (F6) {
           acc[q][qq][0] = v4f{(float)-c.x, (float)-c.x, (float)-c.x, (float)-c.x};
           acc[q][qq][1] = v4f{(float)-c.y, (float)-c.y, (float)-c.y, (float)-c.y};
         } else {
@@ -323,33 +320,28 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
         const int s2 = s + RP;
         ring[q] = s2 >= KQ ? load_step(pnext, s2 - KQ) : load_step(pcur, s2);
       };
-      if constexpr (F8) {
-        // Steps of 64 dishes: the operands of lane (kg, col) are the two
-        // 32-dish operands of steps 2 h and 2 h + 1 side by side (32 bytes),
-        // against the two weight operands of the group side by side.
+      if constexpr (F6) {
+        // Steps of 64 dishes: the E loads of the steps 2 h and 2 h + 1 give,
+        // per polarization, the two variants v2, v1 (6 dwords each,
+        // pack_e_f6) against the group's 6-dword weight operand (dwords 0..3
+        // in entry 2 h of the LDS table, 4..5 in entry 2 h + 1).
         constexpr int KH = KQ / 2;
-        auto cat = [](v4i a, v4i b) -> v8i { return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7); };
-        auto operands8 = [&](v4i e0, v4i e1, v8i (&a1)[2], v8i (&a2)[2]) {
+        auto operands6 = [&](v4i e0, v4i e1, v6i (&o2)[2], v6i (&o1)[2]) {
 #pragma unroll
-          for (int q = 0; q < 2; ++q) {
-            const v8i u0 = unpack_e_pol(e0, q ? 0x07050301u : 0x06040200u);
-            const v8i u1 = unpack_e_pol(e1, q ? 0x07050301u : 0x06040200u);
-            a1[q] = cat(op_v1(u0), op_v1(u1));
-            a2[q] = cat(op_v2(u0), op_v2(u1));
-          }
+          for (int q = 0; q < 2; ++q) pack_e_f6(e0, e1, q ? 0x01010101u : 0u, o2[q], o1[q]);
         };
-        // Weight operands in flight: group g = h NQ + qq is read into slot
-        // qq % WF right after the MFMAs of group g - WF.
+        auto ext = [](v6i a) -> v8i { return __builtin_shufflevector(a, a, 0, 1, 2, 3, 4, 5, -1, -1); };
         constexpr int WF = NT == 3 ? NQ / 2 : NQ;
         static_assert(NQ % WF == 0, "weight slot of group g is qq % WF");
         constexpr int NGF = KH * NQ;
-        auto wload = [&](int g, v4i &w0, v4i &w1) {
+        auto wload = [&](int g, v4i &w0, v2i &w1) {
           const int hh = g / NQ, qq = g - hh * NQ;
           w0 = Wl[(2 * hh * NQ + qq) * 64 + lane];
-          w1 = Wl[((2 * hh + 1) * NQ + qq) * 64 + lane];
+          w1 = *reinterpret_cast<const v2i *>(&Wl[((2 * hh + 1) * NQ + qq) * 64 + lane]);
         };
-        v8i a1c[2], a2c[2];
-        v4i wc0[WF], wc1[WF];
+        v6i o2c[2], o1c[2];
+        v4i wc0[WF];
+        v2i wc1[WF];
         {
           stage_step(0);
           const v4i r0 = staged;
@@ -357,11 +349,11 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
           const v4i r1 = staged;
 #pragma unroll
           for (int g = 0; g < WF; ++g) wload(g, wc0[g], wc1[g]);
-          operands8(r0, r1, a1c, a2c);
+          operands6(r0, r1, o2c, o1c);
         }
 #pragma unroll
         for (int h = 0; h < KH; ++h) {
-          v8i a1n[2], a2n[2];
+          v6i o2n[2], o1n[2];
           v4i rn0, rn1;
           if (h + 1 < KH) {
             stage_step(2 * h + 2);
@@ -372,24 +364,26 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
 #pragma unroll
           for (int qq = 0; qq < NQ; ++qq) {
             const int g = h * NQ + qq;
-            const v8i w = cat(wc0[qq % WF], wc1[qq % WF]);
+            const v4i wx = wc0[qq % WF];
+            const v2i wy = wc1[qq % WF];
+            const v8i w = v8i{wx.x, wx.y, wx.z, wx.w, wy.x, wy.y, 0, 0};
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-              acc[q][qq][0] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a1c[q], w, acc[q][qq][0], 0, 1, 0, 127,
-                                                                               0, 127);
-              acc[q][qq][1] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a2c[q], w, acc[q][qq][1], 0, 1, 0, 127,
-                                                                               0, 127);
+              acc[q][qq][0] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(ext(o1c[q]), w, acc[q][qq][0], 2, 3, 0,
+                                                                               130, 0, 127);
+              acc[q][qq][1] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(ext(o2c[q]), w, acc[q][qq][1], 2, 3, 0,
+                                                                               130, 0, 127);
             }
             if (g + WF < NGF) wload(g + WF, wc0[qq % WF], wc1[qq % WF]);
             __builtin_amdgcn_sched_barrier(0);
           }
-          if (h + 1 < KH) operands8(rn0, rn1, a1n, a2n);
+          if (h + 1 < KH) operands6(rn0, rn1, o2n, o1n);
           __builtin_amdgcn_sched_barrier(0);
           if (h + 1 < KH) {
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-              a1c[q] = a1n[q];
-              a2c[q] = a2n[q];
+              o2c[q] = o2n[q];
+              o1c[q] = o1n[q];
             }
           }
         }

@@ -9,7 +9,8 @@ one form_beams launch over the whole chunk (J = 201 MB written).
 
 Prints one JSON line (stdout) with:
   value          algorithmic int8 TOPS (8 T F B D P ops per step)
-  roofline       dominant kernel vs the MI355X dense int8 peak
+  roofline       dominant kernel vs its binding roofline: HBM at 8 TB/s (the
+                 fp6 MFMA peak, 10.1 PFLOP/s, would allow twice the rate)
   cpu_baseline   the oracle (CPU restatement of the reference cpu.cxx)
                  on a bounded sample of the same workload on this host
                  (kind "port"), and beside it, under reference_cpu_cxx,
@@ -39,6 +40,7 @@ import time
 import numpy as np
 
 MI355X_INT8_TOPS = 5000.0  # dense int8 and fp8 MFMA, 5.0 POP/s
+MI355X_FP6_TOPS = 10100.0  # dense fp6 and fp4 MFMA, 10.1 PFLOP/s
 MI355X_HBM_GBPS = 8000.0   # HBM3E, 8 TB/s
 
 
@@ -761,10 +763,13 @@ def main():
     if args.power_seconds > 0:
         power = power_sample(lambda: plan.form_beams_device(dJ.data_ptr(), dE.data_ptr(), sptr), args.power_seconds)
         if power and power.get("gfx_clock_mhz_mean"):
-            # the dense peak scales with the engine clock (5.0 POP/s at 2.4 GHz)
-            peak_at_clock = MI355X_INT8_TOPS * power["gfx_clock_mhz_mean"] / power["gfx_clock_mhz_max"]
-            power["int8_peak_tops_at_sampled_clock"] = round(peak_at_clock, 1)
-            power["frac_of_peak_at_sampled_clock"] = round(ops / (power["ms_per_launch"] * 1e-3) / 1e12 / peak_at_clock, 4)
+            # the dense peaks scale with the engine clock (fp6 10.1 and int8
+            # 5.0 POP/s at 2.4 GHz)
+            scale = power["gfx_clock_mhz_mean"] / power["gfx_clock_mhz_max"]
+            rate = ops / (power["ms_per_launch"] * 1e-3) / 1e12
+            power["fp6_peak_tops_at_sampled_clock"] = round(MI355X_FP6_TOPS * scale, 1)
+            power["frac_of_fp6_peak_at_sampled_clock"] = round(rate / (MI355X_FP6_TOPS * scale), 4)
+            power["frac_of_int8_peak_at_sampled_clock"] = round(rate / (MI355X_INT8_TOPS * scale), 4)
         log(f"power: {power}")
 
     batched = serving = ingest = others = host_path = None
@@ -822,17 +827,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp8",
+        "dtype": "fp6",
         "data": "synthetic (seeded uniform ucomplex4 E and A in [-7,7]^2, reference gain formula)",
         "config": {"workload": "full-size single shot", "ntimes": T, "nfrequencies": F, "ndishes": D,
                    "npolarizations": 2, "nbeams": B, "j_layout": "[b][f][p][t]"},
-        "roofline": {"bound": "mfma", "achieved": round(tops, 2), "peak": MI355X_INT8_TOPS, "unit": "TFLOP/s",
-                     "frac": round(tops / MI355X_INT8_TOPS, 4), "traffic": traffic,
-                     "note": "MFMA ops per second (fp8 e4m3 x e5m2 operands holding the int4 values exactly, "
-                             "fp32 accumulation of exact integer sums) vs the 5.0 POP/s dense fp8 (= int8) peak; "
-                             "achieved = algorithmic "
-                             "ops per launch / mean launch time (HIP events on the launch stream); traffic = "
-                             "HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated"},
+        "roofline": {"bound": "hbm", "achieved": round(gbps, 1), "peak": MI355X_HBM_GBPS, "unit": "GB/s",
+                     "frac": round(gbps / MI355X_HBM_GBPS, 4), "traffic": traffic,
+                     "mfma": {"achieved_tops": round(tops, 2), "frac_of_fp6_peak": round(tops / MI355X_FP6_TOPS, 4),
+                              "frac_of_int8_peak": round(tops / MI355X_INT8_TOPS, 4)},
+                     "note": "the MFMAs run fp6 operands (e2m3 x e3m2 holding the int4 values exactly, fp32 "
+                             "accumulation of exact integer sums), whose dense peak is 10.1 PFLOP/s: the "
+                             "algorithmic ops need 82 us at that peak and the algorithmic bytes (E + A + G + J) "
+                             "160 us at 8 TB/s, so HBM is the binding roofline; achieved = algorithmic bytes per "
+                             "launch / mean launch time (HIP events on the launch stream); mfma gives the "
+                             "algorithmic int8-equivalent ops per second against the fp6 peak and, for "
+                             "comparison with the int8 form, the 5.0 POP/s int8 peak; traffic = HBM bytes per "
+                             "launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated"},
         "beam_samples_per_s": B * F * T * 2 / (ms * 1e-3),
         "hbm_gbps_algorithmic": round(gbps, 1),
         "hbm_frac": round(gbps / MI355X_HBM_GBPS, 4),

@@ -108,7 +108,7 @@ int bf_plan_describe(const bf_plan *plan, char *buf, size_t len);
  * cuda.cu:35-253; J is fully overwritten. A plan is immutable while in use,
  * so calls on different streams may run concurrently (distinct J buffers).
  * Shapes with P = 2, D a multiple of 8 up to 1024 and gains G with
- * |G| * 128 D < 2^30 run the MFMA kernel (fp8 operands that hold the int4
+ * |G| * 128 D < 2^30 run the MFMA kernel (fp6 operands that hold the int4
  * values exactly, for 33 to 512 dishes; int8 otherwise) at any alignment of
  * E and J;
  * other shapes and gains run a one-thread-per-output kernel with the same

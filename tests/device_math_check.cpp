@@ -102,49 +102,80 @@ void check_operands() {
   }
 }
 
-// OCP 8-bit float formats (the operands of v_mfma_scale_f32_16x16x128_f8f6f4
-// with format codes 0 and 1), decoded from the bit fields: E4M3 (exponent
-// bias 7, 3 mantissa bits, subnormals at exponent 0) and E5M2 (bias 15, 2
-// mantissa bits). NaN and infinity codes are never produced here.
-double dec_e4m3(uint8_t x) {
-  const int s = x >> 7, e = (x >> 3) & 15, m = x & 7;
-  const double v = e == 0 ? std::ldexp(m / 8.0, 1 - 7) : std::ldexp(1.0 + m / 8.0, e - 7);
+// OCP 6-bit float formats (format codes 2 and 3 of v_mfma_scale_f32_16x16x128_f8f6f4): E2M3
+// (exponent bias 1, 3 mantissa bits) and E3M2 (bias 3, 2 mantissa bits);
+// neither has infinities or NaNs.
+double dec_e2m3(uint32_t x) {
+  const int s = (x >> 5) & 1, e = (x >> 3) & 3, m = x & 7;
+  const double v = e == 0 ? m / 8.0 : std::ldexp(1.0 + m / 8.0, e - 1);
   return s ? -v : v;
 }
-double dec_e5m2(uint8_t x) {
-  const int s = x >> 7, e = (x >> 2) & 31, m = x & 3;
-  const double v = e == 0 ? std::ldexp(m / 4.0, 1 - 15) : std::ldexp(1.0 + m / 4.0, e - 15);
+double dec_e3m2(uint32_t x) {
+  const int s = (x >> 5) & 1, e = (x >> 2) & 7, m = x & 3;
+  const double v = e == 0 ? std::ldexp(m / 4.0, 1 - 3) : std::ldexp(1.0 + m / 4.0, e - 3);
   return s ? -v : v;
+}
+uint32_t field6(const uint32_t w[6], int i) { // field i (0..31) of a 6-dword tuple
+  const int bit = 6 * i, wd = bit >> 5, o = bit & 31;
+  uint64_t v = w[wd] >> o;
+  if (o > 26) v |= (uint64_t)w[wd + 1] << (32 - o);
+  return (uint32_t)(v & 63u);
 }
 
-// The fp8 form: the E operand bytes (nibble values 0..15) read as E4M3 and
-// the weight bytes of pack_weights(a, true) read as E5M2; every product is
-// an integer and the dot product equals the int8 form's, so the same
-// corrections give the exact raw sums.
-void check_fp8_operands() {
-  for (int n = 0; n < 16; ++n) expect(dec_e4m3((uint8_t)n) * 512.0 == n, "e4m3 of a nibble byte", n, 0);
-  for (int w = -8; w <= 7; ++w)
-    expect(dec_e5m2(e5m2_times512(w)) == 512.0 * w, "e5m2 of 512 w", (long long)dec_e5m2(e5m2_times512(w)), 512 * w);
-  for (int trial = 0; trial < 20000; ++trial) {
-    uint8_t e[16], a[8];
-    for (int i = 0; i < 16; ++i) e[i] = (uint8_t)next();
-    for (int i = 0; i < 8; ++i) a[i] = (uint8_t)next();
+// The fp6 form: per polarization, the E operand tuples of pack_e_f6 read as
+// E2M3 (times the A scale 2^3) against the weight tuple of pack_weights_f6
+// read as E3M2 give the exact raw sums plus the same nibble-offset
+// corrections; every one of the 16 dishes is used once. Also checks the
+// codes themselves and that the fields' top bits stay clear.
+void check_fp6_operands() {
+  for (int n = 0; n < 16; ++n) expect(dec_e2m3((uint32_t)n) * 8.0 == n, "e2m3 of a nibble", n, 0);
+  for (int w = -8; w <= 8; ++w) expect(dec_e3m2(e3m2_code(w)) == (double)w, "e3m2 of w", (long long)dec_e3m2(e3m2_code(w)), w);
+  int seen[16] = {0};
+  for (int j = 0; j < 16; ++j) ++seen[f6_dish(j)];
+  for (int d = 0; d < 16; ++d) expect(seen[d] == 1, "fp6 dish used once", seen[d], 1);
+  for (int trial = 0; trial < 40000; ++trial) {
+    uint8_t e[32], a[16];
+    for (int i = 0; i < 32; ++i) e[i] = (uint8_t)next();
+    for (int i = 0; i < 16; ++i) a[i] = (uint8_t)next();
     if (trial < 256) {
       std::memset(e, trial, sizeof e);
       std::memset(a, trial & 1 ? 0x00 : 0xff, sizeof a);
     }
-    uint32_t w[4];
-    std::memcpy(w, e, 16);
-    const v4i ev{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
-    const v4i W8 = pack_weights(a), WF = pack_weights(a, true);
+    uint32_t ew[8];
+    std::memcpy(ew, e, 32);
+    const v4i e0{(int)ew[0], (int)ew[1], (int)ew[2], (int)ew[3]}, e1{(int)ew[4], (int)ew[5], (int)ew[6], (int)ew[7]};
+    uint32_t W[6];
+    pack_weights_f6(a, a + 8, W);
+    int sum_ar = 0, sum_ai = 0;
+    for (int d = 0; d < 16; ++d) {
+      sum_ar += bfo_dec_re(a[d]);
+      sum_ai += bfo_dec_im(a[d]);
+    }
+    int corr_re, corr_im;
+    corrections(sum_ar, sum_ai, &corr_re, &corr_im);
     for (int p = 0; p < 2; ++p) {
-      const v8i u = unpack_e_pol(ev, p ? 0x07050301u : 0x06040200u);
+      v6i v2, v1;
+      pack_e_f6(e0, e1, (uint32_t)p * 0x01010101u, v2, v1);
+      int want_re = 0, want_im = 0;
+      for (int d = 0; d < 16; ++d) {
+        const int er = bfo_dec_re(e[2 * d + p]), ei = bfo_dec_im(e[2 * d + p]);
+        const int ar = bfo_dec_re(a[d]), ai = bfo_dec_im(a[d]);
+        want_re += ar * er - ai * ei;
+        want_im += ar * ei + ai * er;
+      }
       for (int variant = 0; variant < 2; ++variant) {
-        const v4i op = variant ? op_v2(u) : op_v1(u);
+        const v6i op = variant ? v2 : v1;
+        uint32_t ow[6];
+        for (int i = 0; i < 6; ++i) ow[i] = (uint32_t)op[i];
         double f = 0;
-        for (int i = 0; i < 16; ++i) f += dec_e4m3((uint8_t)byte_of(op, i)) * dec_e5m2((uint8_t)byte_of(WF, i));
-        const int want = dot16(op, W8);
-        expect(f == (double)want, "fp8 dot product", (long long)f, want);
+        for (int i = 0; i < 32; ++i) {
+          const uint32_t c = field6(ow, i);
+          expect(c < 16, "fp6 E field top bits clear", c, 15);
+          f += dec_e2m3(c) * 8.0 * dec_e3m2(field6(W, i));
+        }
+        const int got = (int)f - (variant ? corr_im : corr_re);
+        const int want = variant ? want_im : want_re;
+        expect(f == std::floor(f) && got == want, variant ? "fp6 raw im" : "fp6 raw re", got, want);
       }
     }
   }
@@ -211,7 +242,7 @@ int main() {
   check_codec();
   check_perm();
   check_operands();
-  check_fp8_operands();
+  check_fp6_operands();
   check_requant();
   check_packing();
   if (failures) {
