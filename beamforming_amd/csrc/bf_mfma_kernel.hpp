@@ -160,7 +160,12 @@ __device__ __forceinline__ void stage_weights(const v4i *W, const int *corr, con
     if (N % kThreads == 0 || i < N) Wl[i] = tmp[k];
   }
   if (cg) {
-    Cl[threadIdx.x] = cv;
+    // fp6 accumulators are floats: their start values, minus the offsets
+    // (exact: |offset| < 2^24), are stored ready to use
+    if constexpr (mfma_f6(KS))
+      Cl[threadIdx.x] = make_int2(__float_as_int((float)-cv.x), __float_as_int((float)-cv.y));
+    else
+      Cl[threadIdx.x] = cv;
     Gl[threadIdx.x] = gv;
   }
 }
@@ -286,8 +291,9 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         if constexpr (F6) {
-          acc[q][qq][0] = v4f{(float)-c.x, (float)-c.x, (float)-c.x, (float)-c.x};
-          acc[q][qq][1] = v4f{(float)-c.y, (float)-c.y, (float)-c.y, (float)-c.y};
+          const float cr = __int_as_float(c.x), ci = __int_as_float(c.y);
+          acc[q][qq][0] = v4f{cr, cr, cr, cr};
+          acc[q][qq][1] = v4f{ci, ci, ci, ci};
         } else {
           acc[q][qq][0] = v4i{-c.x, -c.x, -c.x, -c.x};
           acc[q][qq][1] = v4i{-c.y, -c.y, -c.y, -c.y};
