@@ -241,6 +241,29 @@ def test_nibble_minus_eight_inputs():
     check_full(shape, E, A, G)
 
 
+@pytest.mark.parametrize("D", [512, 200, 64])
+@pytest.mark.parametrize("a_byte,e_kind", [(0x00, "random"), (0xFF, "random"), (0x0F, "random"), (0xF0, "random"),
+                                           (0x00, 0x00), (0xFF, 0x00), (0x00, 0xFF), (0x0F, 0xF0)])
+def test_extreme_operands(D, a_byte, e_kind):
+    # The MFMA operands at their extremes: weights -8 and 7 in every field
+    # position of the fp6 operands (E3M2 codes with and without sign), E
+    # nibbles 0 and 15 in every position (E2M3 codes 0 and 15 after the
+    # complement of one variant), at 64 and 512 dishes and with a dish tail
+    # (200). The raw sums reach 128 D in magnitude; the gains keep them in
+    # the requantisation's range and away from saturation.
+    T, F, B = 96, 2, 40
+    shape = bf.Shape.make(T, F, D, B)
+    rng = np.random.default_rng(D + a_byte)
+    if e_kind == "random":
+        E = rng.integers(0, 256, T * F * D * 2, dtype=np.uint8)
+    else:
+        E = np.full(T * F * D * 2, e_kind, np.uint8)
+    A = np.full(F * B * D, a_byte, np.uint8)
+    A[rng.integers(0, A.size, A.size // 7)] = rng.integers(0, 256, A.size // 7, dtype=np.uint8)
+    G = (rng.uniform(0.5, 7.0, F * B) / (16.0 * D)).astype(np.float32)
+    check_full(shape, E, A, G)
+
+
 def test_impulse():
     T, F, D, B = 32, 3, 64, 16
     shape = bf.Shape.make(T, F, D, B)
