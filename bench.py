@@ -18,12 +18,13 @@ Prints one JSON line (stdout) with:
                  compile-time shape on this host (kind "reference")
   beam_samples_per_s, hbm_gbps, parity of the sampled frequencies
   power_and_clock  socket power and engine clocks (amd-smi) while the
-                 full-size launch runs back to back, and the int8 peak at
-                 that clock
+                 full-size launch runs back to back, and the fp6 and int8
+                 peaks at that clock
   host_path      the copy-in / launch / copy-out entry point on pinned host
                  buffers (PCIe-inclusive, never the value)
   other_configs  the small and wide configurations and the full-size shape
-                 with the J2 output layout, timed the same way
+                 with the J2 output layout, timed the same way, each with
+                 its binding roofline (fp6 MFMA or HBM)
   batched_vs_plain_launch, serving, ingest   the batching, serving and E
                  ingest paths on their own workloads
 
@@ -54,6 +55,19 @@ def workload_bytes(T, F, D, B, P=2):
 
 def workload_ops(T, F, D, B, P=2):
     return 8 * T * F * B * D * P
+
+
+def roofline(ops, nbytes, ms):
+    """The binding roofline of a launch of `ops` algorithmic int8-equivalent
+    operations and `nbytes` algorithmic HBM bytes taking `ms`: the larger of
+    the fp6 MFMA time (the kernels' MFMA form) and the 8 TB/s HBM time, and
+    the launch's fraction of it, with both fractions and the int8 one beside."""
+    t_mfma = ops / (MI355X_FP6_TOPS * 1e12)
+    t_hbm = nbytes / (MI355X_HBM_GBPS * 1e9)
+    t = ms * 1e-3
+    return {"bound": "mfma_fp6" if t_mfma > t_hbm else "hbm", "frac": round(max(t_mfma, t_hbm) / t, 4),
+            "frac_mfma_fp6": round(t_mfma / t, 4), "frac_hbm": round(t_hbm / t, 4),
+            "frac_mfma_int8": round(ops / (MI355X_INT8_TOPS * 1e12) / t, 4)}
 
 
 def host_uniform(bf, torch, n, seed):
@@ -326,7 +340,7 @@ def run_batched(bf, torch, stream, steps, warmup, seed=7, C=64, D=512, B=96):
         "workload": f"batched: {C} channels, D={D}, B={B}, P=2, ntimes log-uniform [512, 32768] (seed {seed}), "
                     f"sum ntimes={sumT}",
         "tops": round(ops / (best * 1e-3) / 1e12, 2),
-        "roofline_frac_mfma": round(ops / (best * 1e-3) / 1e12 / MI355X_INT8_TOPS, 4),
+        "roofline": roofline(ops, sumT * D * 2 + C * B * D + 4 * C * B + sumT * B * 2, best),
         "hbm_gbps_algorithmic": round((sumT * D * 2 + C * B * D + 4 * C * B + sumT * B * 2) / (best * 1e-3) / 1e9, 1),
         "beam_samples_per_s": sumT * B * 2 / (best * 1e-3),
         "ms_persistent": round(ms_batch, 4),
@@ -401,7 +415,7 @@ def run_other_configs(bf, torch, stream, steps, warmup, names=("small", "wide", 
             "weights": "phase ramp" if ramp else "seeded uniform",
             "us_per_launch": round(ms * 1e3, 2),
             "tops": round(tops, 2),
-            "roofline_frac_mfma": round(tops / MI355X_INT8_TOPS, 4),
+            "roofline": roofline(workload_ops(T, F, D, B), workload_bytes(T, F, D, B), ms),
             "hbm_gbps_algorithmic": round(workload_bytes(T, F, D, B) / (ms * 1e-3) / 1e9, 1),
             "beam_samples_per_s": B * F * T * 2 / (ms * 1e-3),
             "kernel": desc,

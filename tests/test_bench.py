@@ -51,3 +51,12 @@ def test_synthetic_inputs_match_the_oracle_generators():
     for n, D in [(32 * 96, 512), (64 * 96, 512), (16 * 512, 512), (7, 3)]:
         assert np.array_equal(bench.reference_gains(n, D), bo.setup_gain(n, D))
     assert np.array_equal(bench.phase_ramp_weights(16, 512, 512), bo.phase_ramp(16, 512, 512))
+
+
+def test_roofline_picks_the_binding_bound():
+    # full size: HBM binds (1.28 GB at 8 TB/s = 160 us against 82 us of fp6 MFMA)
+    r = bench.roofline(bench.workload_ops(32768, 32, 512, 96), bench.workload_bytes(32768, 32, 512, 96), 0.3192)
+    assert r["bound"] == "hbm" and abs(r["frac"] - 0.5) < 0.001 and r["frac"] == r["frac_hbm"]
+    # wide beams: the fp6 MFMA bound binds (550 G ops at 10.1 POP/s = 54 us against 34 us of HBM)
+    r = bench.roofline(bench.workload_ops(8192, 16, 512, 512), bench.workload_bytes(8192, 16, 512, 512), 0.1)
+    assert r["bound"] == "mfma_fp6" and r["frac"] == r["frac_mfma_fp6"] and r["frac_mfma_int8"] > r["frac"]
