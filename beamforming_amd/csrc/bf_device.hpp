@@ -334,6 +334,19 @@ BF_HD int64_t e_col_offset(int j, int piece, int D, bool dt) {
 // ([b][f][t][p]).
 BF_HD int64_t j_row_bfpt(int b, int F, int f, int p, int64_t T) { return (((int64_t)b * F + f) * 2 + p) * T; }
 BF_HD int64_t j_row_bftp(int b, int F, int f, int64_t T) { return ((int64_t)b * F + f) * T * 2; }
+// LDS staging area of J[b][f][p][t]: two slots (even and odd rounds) of
+// 2 NT x 32 rows x 128 bytes, rows in (beam, polarization) order, the 16-byte
+// pieces of a row rotated by its beam (which spreads the epilogue's writes of
+// 16 beams over the LDS banks). stage_word: the word of lane kg of `wave`
+// (times 16 wave + 4 kg .. + 3 of the round) in row srow; stage_piece_byte:
+// the byte of piece seg (times 16 seg .. + 15 of the round) of row srow.
+BF_HD int stage_word(int nt, int slot, int srow, int wave, int kg) {
+  return slot * nt * 64 * 32 + srow * 32 + (wave ^ ((srow >> 1) & 7)) * 4 + kg;
+}
+BF_HD int stage_piece_byte(int nt, int slot, int srow, int seg) {
+  return 16 * (slot * nt * 64 * 8 + srow * 8 + (seg ^ ((srow >> 1) & 7)));
+}
+
 // Tiles [begin, end) of time chunk `chunk` of a plain launch.
 BF_HD void chunk_tiles(int chunk, int tiles_per_chunk, int64_t ntiles, int64_t *begin, int64_t *end) {
   *begin = (int64_t)chunk * tiles_per_chunk;

@@ -461,16 +461,12 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
     if constexpr (JL == 0) {
       if (Sl) {
         // Row (beam qq * 16 + col, polarization q), bytes 4 kg .. 4 kg + 3 of
-        // the wave's 16-byte piece of the round's 128 bytes; piece slots are
-        // rotated by the row (as the cooperative store below reads them).
-        uint32_t *buf = reinterpret_cast<uint32_t *>(Sl + (size_t)(r & 1) * (NT * 64 * 8));
+        // the wave's 16-byte piece of the round's 128 bytes (see stage_word).
+        uint32_t *buf = reinterpret_cast<uint32_t *>(Sl);
 #pragma unroll
         for (int qq = 0; qq < NQ; ++qq)
 #pragma unroll
-          for (int q = 0; q < 2; ++q) {
-            const int srow = (qq * 16 + cx) * 2 + q;
-            buf[srow * 32 + (wave ^ ((srow >> 1) & 7)) * 4 + kg] = wv[q][qq];
-          }
+          for (int q = 0; q < 2; ++q) buf[stage_word(NT, (int)(r & 1), (qq * 16 + cx) * 2 + q, wave, kg)] = wv[q][qq];
       } else {
         // The E loads of the next tile's first steps were issued thousands
         // of cycles ago; completing them here keeps the stores (counted by
@@ -536,7 +532,8 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
       for (int k = 0; k < NT; ++k) {
         const int q = threadIdx.x + kThreads * k;
         const int srow = q >> 3, seg = q & 7;
-        const v4i v = buf[srow * 8 + (seg ^ ((srow >> 1) & 7))];
+        const v4i v = *reinterpret_cast<const v4i *>(reinterpret_cast<const uint8_t *>(Sl) +
+                                                     stage_piece_byte(NT, (int)(r & 1), srow, seg));
         const int b = job.bg * NT * 32 + (srow >> 1);
         const int64_t t = t_round + 16 * seg;
         if (b < job.B && t < t_end)

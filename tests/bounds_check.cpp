@@ -7,6 +7,11 @@
 //  * every 16-byte E load lies inside the E buffer of its launch or chunk,
 //  * every J store lies inside the J buffer of its launch or chunk,
 //  * every J byte is written exactly once (nothing missed, nothing twice),
+//  * with the LDS staging area and [b][f][p][t], every staging write and
+//    read stays in its slot's row, and every J byte stored comes from the
+//    staging byte that holds that time of that job (the staging area is
+//    modelled as per-byte tags of job and time, written as the epilogue
+//    writes it, and kept from job to job as the batch kernel's LDS is),
 // for both J layouts, with and without the LDS staging area, with and
 // without a dish tail, all beam tilings (NT = 1..3) and grids. The loop
 // structure restates run_tiles in bf_mfma_kernel.hpp: its prologue loads the
@@ -48,6 +53,13 @@ struct Buffers {
   std::vector<uint8_t> jcount;
 };
 
+// A workgroup's J staging area: per byte, the tag (job serial << 32 | time)
+// of what the epilogue last wrote there.
+struct Staging {
+  std::vector<int64_t> tag;
+  int64_t serial = 0;
+};
+
 struct Tiling {
   int nt, ks, nbg;
   bool dt, staged;
@@ -56,8 +68,8 @@ struct Tiling {
 
 // run_tiles for one job (a frequency of a launch or chunk, one beam group,
 // tiles [tile_first, tile_end)); e_base is the job's E offset in the buffer.
-void run_job(const Tiling &c, Buffers &buf, int64_t e_base, int64_t e_tstride, int64_t T, int F, int f, int D, int B,
-             int bg, int64_t tile_first, int64_t tile_end) {
+void run_job(const Tiling &c, Buffers &buf, Staging &lds, int64_t e_base, int64_t e_tstride, int64_t T, int F, int f,
+             int D, int B, int bg, int64_t tile_first, int64_t tile_end) {
   const int KQ = c.ks / 2;            // 32-dish K steps
   const int RP0 = c.nt >= 3 ? 4 : 8;  // E prefetch distance of run_tiles
   const int RP = RP0 < KQ ? RP0 : KQ;
@@ -83,6 +95,28 @@ void run_job(const Tiling &c, Buffers &buf, int64_t e_base, int64_t e_tstride, i
   // store_piece / store_row16: 16 bytes when whole, else bytes i < n
   auto piece16 = [&](int64_t off, int n) { store(off, fast_store || n >= 16 ? 16 : (n > 0 ? n : 0)); };
 
+  const int64_t serial = ++lds.serial;
+  auto tag_of = [&](int64_t t) { return serial << 32 | t; };
+  const int64_t stage_bytes = (int64_t)mfma_lds_staging(c.nt);
+  if ((int64_t)lds.tag.size() < stage_bytes) lds.tag.assign((size_t)stage_bytes, -1);
+  // a row's bytes in one slot: [lo, lo + 128)
+  auto row_lo = [&](int slot, int srow) { return ((int64_t)slot * c.nt * 64 + srow) * 128; };
+  // the epilogue's staging writes of one tile ([b][f][p][t])
+  auto stage_tile = [&](int64_t r, int wave, int64_t tile) {
+    const int slot = (int)(r & 1);
+    for (int lane = 0; lane < 64; ++lane) {
+      const int kg = lane >> 4, col = lane & 15;
+      for (int qq = 0; qq < NQ; ++qq)
+        for (int q = 0; q < 2; ++q) {
+          const int srow = (qq * 16 + col) * 2 + q;
+          const int64_t at = 4 * (int64_t)stage_word(c.nt, slot, srow, wave, kg);
+          expect(at >= row_lo(slot, srow) && at + 4 <= row_lo(slot, srow) + 128 && at + 4 <= stage_bytes,
+                 "staging write inside its row", at, srow);
+          for (int i = 0; i < 4; ++i) lds.tag[(size_t)(at + i)] = tag_of(tile * kTileT + 4 * kg + i);
+        }
+    }
+  };
+
   for (int wave = 0; wave < kWaves; ++wave)
     for (int lane = 0; lane < 64; ++lane)
       for (int q = 0; q < RP; ++q) load(tile_first + wave, lane, q); // prologue
@@ -99,7 +133,10 @@ void run_job(const Tiling &c, Buffers &buf, int64_t e_base, int64_t e_tstride, i
           else
             load(tile, lane, j2);
         }
-      if (c.staged) continue;
+      if (c.staged) {
+        if (c.jl == 0) stage_tile(r, wave, tile);
+        continue;
+      }
       const int64_t t0 = tile * kTileT;
       const int nvalid = T - t0 < kTileT ? (int)(T - t0) : kTileT;
       for (int lane = 0; lane < 64; ++lane) {
@@ -127,9 +164,19 @@ void run_job(const Tiling &c, Buffers &buf, int64_t e_base, int64_t e_tstride, i
         const int q = thread + kThreads * k;
         if (c.jl == 0) {
           const int srow = q >> 3, seg = q & 7;
+          const int slot = (int)(r & 1);
+          const int64_t at = stage_piece_byte(c.nt, slot, srow, seg);
+          expect(at >= row_lo(slot, srow) && at + 16 <= row_lo(slot, srow) + 128 && at + 16 <= stage_bytes,
+                 "staging read inside its row", at, srow);
           const int b = bg * c.nt * 32 + (srow >> 1);
           const int64_t t = t_round + 16 * seg;
-          if (b < B && t < t_end) piece16(j_row_bfpt(b, F, f, srow & 1, T) + t, t_end - t < 16 ? (int)(t_end - t) : 16);
+          if (b < B && t < t_end) {
+            const int n = t_end - t < 16 ? (int)(t_end - t) : 16;
+            for (int i = 0; i < n; ++i)
+              expect(lds.tag[(size_t)(at + i)] == tag_of(t + i), "J byte from the staging byte of its time", t + i,
+                     lds.tag[(size_t)(at + i)] & 0xffffffffll);
+            piece16(j_row_bfpt(b, F, f, srow & 1, T) + t, n);
+          }
         } else {
           const int srow = q >> 4, seg = q & 15;
           const int b = bg * c.nt * 32 + srow;
@@ -176,12 +223,13 @@ void check_plain(int id) {
   Buffers buf;
   buf.e_bytes = T * F * D * 2;
   buf.jcount.assign((size_t)(B * F * 2 * T), 0);
+  Staging lds;
   for (int f = 0; f < F; ++f)
     for (int bg = 0; bg < c.nbg; ++bg)
       for (int ch = 0; ch < nchunks; ++ch) {
         int64_t tb, te;
         chunk_tiles(ch, tpc, ntiles, &tb, &te);
-        run_job(c, buf, (int64_t)f * D * 2, (int64_t)F * D * 2, T, F, f, D, B, bg, tb, te);
+        run_job(c, buf, lds, (int64_t)f * D * 2, (int64_t)F * D * 2, T, F, f, D, B, bg, tb, te);
       }
   check_written_once(buf, "plain launch: every J byte written once", id);
 }
@@ -212,6 +260,7 @@ void check_batch(int id) {
     bufs[i].e_bytes = chunks[i].ntimes * nf * D * 2;
     bufs[i].jcount.assign((size_t)(B * nf * 2 * chunks[i].ntimes), 0);
   }
+  Staging lds;
   // through the segment records the kernel reads (E addresses as offsets
   // from a chunk base that is never dereferenced)
   for (const BatchItem &s : segs) {
@@ -220,7 +269,7 @@ void check_batch(int id) {
     const SegRec r = seg_record(k, s, D);
     expect(r.nf == chunk_nfreq(chunks[s.chunk]) && r.T == k.ntimes && r.channel == k.channel + s.f,
            "segment record fields", s.chunk, s.f);
-    run_job(c, bufs[s.chunk], (int64_t)(r.E - k.E), (int64_t)r.nf * D * 2, r.T, r.nf, r.f_bg & 0xffff, D, B,
+    run_job(c, bufs[s.chunk], lds, (int64_t)(r.E - k.E), (int64_t)r.nf * D * 2, r.T, r.nf, r.f_bg & 0xffff, D, B,
             r.f_bg >> 16, r.tile_begin, r.tile_end);
   }
   for (int i = 0; i < n; ++i) check_written_once(bufs[i], "batched launch: every chunk's J byte written once", id);

@@ -2,8 +2,9 @@
 the tile pipeline's address arithmetic, shared with the kernels through
 bf_device.hpp, run for every wave and lane of plain and batched launches over
 random shapes and chunk lists. Every E load and J store must stay inside its
-buffer and every J byte must be written exactly once. Compiled for the CPU
-with hipcc (no GPU)."""
+buffer, every J byte must be written exactly once, and every J byte that goes
+through the LDS staging area must come from the staging byte holding its
+time. Compiled for the CPU with hipcc (no GPU)."""
 import os
 import shutil
 import subprocess
