@@ -347,6 +347,18 @@ BF_HD int stage_piece_byte(int nt, int slot, int srow, int seg) {
   return 16 * (slot * nt * 64 * 8 + srow * 8 + (seg ^ ((srow >> 1) & 7)));
 }
 
+// LDS staging area of J[b][f][t][p]: two slots of NT x 32 beam rows x 256
+// bytes (8 tiles x 16 times x 2 polarizations), the 16-byte pieces of a row
+// rotated by its beam. stage_pair_bftp: the byte of lane kg's 8 bytes (times
+// 4 kg .. 4 kg + 3 of `wave`'s tile, polarizations interleaved) in row srow;
+// stage_piece_bftp: the byte of piece seg (times 8 seg .. 8 seg + 7).
+BF_HD int stage_pair_bftp(int nt, int slot, int srow, int wave, int kg) {
+  return 8 * (slot * nt * 64 * 16 + srow * 32 + ((2 * wave + (kg >> 1)) ^ (srow & 15)) * 2 + (kg & 1));
+}
+BF_HD int stage_piece_bftp(int nt, int slot, int srow, int seg) {
+  return 16 * (slot * nt * 64 * 8 + srow * 16 + (seg ^ (srow & 15)));
+}
+
 // Tiles [begin, end) of time chunk `chunk` of a plain launch.
 BF_HD void chunk_tiles(int chunk, int tiles_per_chunk, int64_t ntiles, int64_t *begin, int64_t *end) {
   *begin = (int64_t)chunk * tiles_per_chunk;

@@ -493,15 +493,13 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
       }
       if (Sl) {
         // Row = beam, 256 bytes per round (8 tiles x 32 bytes); the wave's
-        // 32 bytes are 16-byte slots 2 wave and 2 wave + 1, rotated by the
-        // row (as the cooperative store below reads them).
-        v2i *buf = reinterpret_cast<v2i *>(Sl + (size_t)(r & 1) * (NT * 64 * 8));
+        // 32 bytes are 16-byte pieces 2 wave and 2 wave + 1 (see
+        // stage_pair_bftp).
+        uint8_t *buf = reinterpret_cast<uint8_t *>(Sl);
 #pragma unroll
-        for (int qq = 0; qq < NQ; ++qq) {
-          const int srow = qq * 16 + cx;
-          const int slot = (2 * wave + (kg >> 1)) ^ (srow & 15);
-          buf[srow * 32 + slot * 2 + (kg & 1)] = v2i{(int)lo[qq], (int)hi[qq]};
-        }
+        for (int qq = 0; qq < NQ; ++qq)
+          *reinterpret_cast<v2i *>(buf + stage_pair_bftp(NT, (int)(r & 1), qq * 16 + cx, wave, kg)) =
+              v2i{(int)lo[qq], (int)hi[qq]};
       } else {
         __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0), as above
 #pragma unroll
@@ -523,7 +521,6 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
     // which every wave reaches after finishing these reads.
     __builtin_amdgcn_s_waitcnt(0xc07f); // lgkmcnt(0)
     __builtin_amdgcn_s_barrier();
-    const v4i *buf = Sl + (size_t)(r & 1) * (NT * 64 * 8);
     const int64_t t_round = (tile_first + r * step) * kTileT;
     const int64_t t_end = tile_end * kTileT < job.T ? tile_end * kTileT : job.T;
     __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0), as above
@@ -547,7 +544,8 @@ __device__ __forceinline__ void run_tiles(const TileJob &job, const int64_t tile
       for (int k = 0; k < NT; ++k) {
         const int q = threadIdx.x + kThreads * k;
         const int srow = q >> 4, seg = q & 15;
-        const v4i v = buf[srow * 16 + (seg ^ (srow & 15))];
+        const v4i v = *reinterpret_cast<const v4i *>(reinterpret_cast<const uint8_t *>(Sl) +
+                                                     stage_piece_bftp(NT, (int)(r & 1), srow, seg));
         const int b = job.bg * NT * 32 + srow;
         const int64_t t = t_round + 8 * seg;
         if (b < job.B && t < t_end)

@@ -7,7 +7,7 @@
 //  * every 16-byte E load lies inside the E buffer of its launch or chunk,
 //  * every J store lies inside the J buffer of its launch or chunk,
 //  * every J byte is written exactly once (nothing missed, nothing twice),
-//  * with the LDS staging area and [b][f][p][t], every staging write and
+//  * with the LDS staging area (either J layout), every staging write and
 //    read stays in its slot's row, and every J byte stored comes from the
 //    staging byte that holds that time of that job (the staging area is
 //    modelled as per-byte tags of job and time, written as the epilogue
@@ -101,11 +101,23 @@ void run_job(const Tiling &c, Buffers &buf, Staging &lds, int64_t e_base, int64_
   if ((int64_t)lds.tag.size() < stage_bytes) lds.tag.assign((size_t)stage_bytes, -1);
   // a row's bytes in one slot: [lo, lo + 128)
   auto row_lo = [&](int slot, int srow) { return ((int64_t)slot * c.nt * 64 + srow) * 128; };
-  // the epilogue's staging writes of one tile ([b][f][p][t])
+  // a [b][f][t][p] beam row's bytes in one slot: [lo, lo + 256)
+  auto beam_lo = [&](int slot, int srow) { return ((int64_t)slot * c.nt * 32 + srow) * 256; };
+  // the epilogue's staging writes of one tile
   auto stage_tile = [&](int64_t r, int wave, int64_t tile) {
     const int slot = (int)(r & 1);
     for (int lane = 0; lane < 64; ++lane) {
       const int kg = lane >> 4, col = lane & 15;
+      if (c.jl == 1) { // 8 bytes per beam: times 4 kg .. 4 kg + 3, polarizations interleaved
+        for (int qq = 0; qq < NQ; ++qq) {
+          const int srow = qq * 16 + col;
+          const int64_t at = stage_pair_bftp(c.nt, slot, srow, wave, kg);
+          expect(at >= beam_lo(slot, srow) && at + 8 <= beam_lo(slot, srow) + 256 && at + 8 <= stage_bytes,
+                 "staging write inside its row", at, srow);
+          for (int i = 0; i < 8; ++i) lds.tag[(size_t)(at + i)] = tag_of(2 * (tile * kTileT + 4 * kg) + i);
+        }
+        continue;
+      }
       for (int qq = 0; qq < NQ; ++qq)
         for (int q = 0; q < 2; ++q) {
           const int srow = (qq * 16 + col) * 2 + q;
@@ -134,7 +146,7 @@ void run_job(const Tiling &c, Buffers &buf, Staging &lds, int64_t e_base, int64_
             load(tile, lane, j2);
         }
       if (c.staged) {
-        if (c.jl == 0) stage_tile(r, wave, tile);
+        stage_tile(r, wave, tile);
         continue;
       }
       const int64_t t0 = tile * kTileT;
@@ -179,9 +191,19 @@ void run_job(const Tiling &c, Buffers &buf, Staging &lds, int64_t e_base, int64_
           }
         } else {
           const int srow = q >> 4, seg = q & 15;
+          const int slot = (int)(r & 1);
+          const int64_t at = stage_piece_bftp(c.nt, slot, srow, seg);
+          expect(at >= beam_lo(slot, srow) && at + 16 <= beam_lo(slot, srow) + 256 && at + 16 <= stage_bytes,
+                 "staging read inside its row", at, srow);
           const int b = bg * c.nt * 32 + srow;
           const int64_t t = t_round + 8 * seg;
-          if (b < B && t < t_end) piece16(j_row_bftp(b, F, f, T) + 2 * t, t_end - t < 8 ? (int)(2 * (t_end - t)) : 16);
+          if (b < B && t < t_end) {
+            const int n = t_end - t < 8 ? (int)(2 * (t_end - t)) : 16;
+            for (int i = 0; i < n; ++i)
+              expect(lds.tag[(size_t)(at + i)] == tag_of(2 * t + i), "J byte from the staging byte of its time", 2 * t + i,
+                     lds.tag[(size_t)(at + i)] & 0xffffffffll);
+            piece16(j_row_bftp(b, F, f, T) + 2 * t, n);
+          }
         }
       }
   }
