@@ -10,7 +10,7 @@
 //  * with the LDS staging area (either J layout), every staging write and
 //    read stays in its slot's row, and every J byte stored comes from the
 //    staging byte that holds that time of that job (the staging area is
-//    modelled as per-byte tags of job and time, written as the epilogue
+//    modelled as per-byte tags of job, row and time, written as the epilogue
 //    writes it, and kept from job to job as the batch kernel's LDS is),
 // for both J layouts, with and without the LDS staging area, with and
 // without a dish tail, all beam tilings (NT = 1..3) and grids. The loop
@@ -21,7 +21,13 @@
 // stores the wave's words directly (4 bytes per lane, polarization and
 // 16-beam group of [b][f][p][t]; 8 bytes per lane and group of [b][f][t][p])
 // or, when staged, stores the whole round's rows cooperatively.
-// Prints "ok" and exits 0 when every check passes.
+// Prints "ok" and exits 0 when every check passes. Built with -DPLANT=n
+// (1..3), the model makes one planted mistake of its own (drops the row
+// clamp at T - 1; reads the other staging slot; writes [b][f][p][t] staging
+// words one row off), which the checks must catch.
+#ifndef PLANT
+#define PLANT 0
+#endif
 #include "bf_items.hpp"
 #include "bf_mfma_kernel.hpp"
 
@@ -53,8 +59,8 @@ struct Buffers {
   std::vector<uint8_t> jcount;
 };
 
-// A workgroup's J staging area: per byte, the tag (job serial << 32 | time)
-// of what the epilogue last wrote there.
+// A workgroup's J staging area: per byte, the tag (job serial, staging row,
+// position in the J row) of what the epilogue last wrote there.
 struct Staging {
   std::vector<int64_t> tag;
   int64_t serial = 0;
@@ -81,8 +87,8 @@ void run_job(const Tiling &c, Buffers &buf, Staging &lds, int64_t e_base, int64_
 
   auto load = [&](int64_t tile, int lane, int j) {
     const int ld_row = lane & 15, ld_piece = lane >> 4;
-    const int64_t off =
-        e_base + 16 * ld_piece + e_row_index(tile, ld_row, T) * e_tstride + e_col_offset(j, ld_piece, D, c.dt);
+    const int64_t row = PLANT == 1 ? tile * kTileT + ld_row : e_row_index(tile, ld_row, T);
+    const int64_t off = e_base + 16 * ld_piece + row * e_tstride + e_col_offset(j, ld_piece, D, c.dt);
     expect(off >= 0 && off + 16 <= buf.e_bytes, "E load inside the buffer", off, buf.e_bytes);
   };
   // n bytes written at off (a piece of n <= 0 bytes writes nothing: its
@@ -96,7 +102,8 @@ void run_job(const Tiling &c, Buffers &buf, Staging &lds, int64_t e_base, int64_
   auto piece16 = [&](int64_t off, int n) { store(off, fast_store || n >= 16 ? 16 : (n > 0 ? n : 0)); };
 
   const int64_t serial = ++lds.serial;
-  auto tag_of = [&](int64_t t) { return serial << 32 | t; };
+  // job, staging row (beam and polarization) and position in the J row
+  auto tag_of = [&](int srow, int64_t t) { return serial << 40 | (int64_t)srow << 24 | t; };
   const int64_t stage_bytes = (int64_t)mfma_lds_staging(c.nt);
   if ((int64_t)lds.tag.size() < stage_bytes) lds.tag.assign((size_t)stage_bytes, -1);
   // a row's bytes in one slot: [lo, lo + 128)
@@ -114,17 +121,16 @@ void run_job(const Tiling &c, Buffers &buf, Staging &lds, int64_t e_base, int64_
           const int64_t at = stage_pair_bftp(c.nt, slot, srow, wave, kg);
           expect(at >= beam_lo(slot, srow) && at + 8 <= beam_lo(slot, srow) + 256 && at + 8 <= stage_bytes,
                  "staging write inside its row", at, srow);
-          for (int i = 0; i < 8; ++i) lds.tag[(size_t)(at + i)] = tag_of(2 * (tile * kTileT + 4 * kg) + i);
+          for (int i = 0; i < 8; ++i) lds.tag[(size_t)(at + i)] = tag_of(srow, 2 * (tile * kTileT + 4 * kg) + i);
         }
         continue;
       }
       for (int qq = 0; qq < NQ; ++qq)
         for (int q = 0; q < 2; ++q) {
           const int srow = (qq * 16 + col) * 2 + q;
-          const int64_t at = 4 * (int64_t)stage_word(c.nt, slot, srow, wave, kg);
-          expect(at >= row_lo(slot, srow) && at + 4 <= row_lo(slot, srow) + 128 && at + 4 <= stage_bytes,
-                 "staging write inside its row", at, srow);
-          for (int i = 0; i < 4; ++i) lds.tag[(size_t)(at + i)] = tag_of(tile * kTileT + 4 * kg + i);
+          const int64_t at = 4 * (int64_t)stage_word(c.nt, slot, PLANT == 3 ? srow ^ 1 : srow, wave, kg);
+          expect(at >= 0 && at + 4 <= stage_bytes, "staging write inside the area", at, srow);
+          for (int i = 0; i < 4; ++i) lds.tag[(size_t)(at + i)] = tag_of(srow, tile * kTileT + 4 * kg + i);
         }
     }
   };
@@ -176,7 +182,7 @@ void run_job(const Tiling &c, Buffers &buf, Staging &lds, int64_t e_base, int64_
         const int q = thread + kThreads * k;
         if (c.jl == 0) {
           const int srow = q >> 3, seg = q & 7;
-          const int slot = (int)(r & 1);
+          const int slot = (int)(r & 1) ^ (PLANT == 2 ? 1 : 0);
           const int64_t at = stage_piece_byte(c.nt, slot, srow, seg);
           expect(at >= row_lo(slot, srow) && at + 16 <= row_lo(slot, srow) + 128 && at + 16 <= stage_bytes,
                  "staging read inside its row", at, srow);
@@ -185,8 +191,8 @@ void run_job(const Tiling &c, Buffers &buf, Staging &lds, int64_t e_base, int64_
           if (b < B && t < t_end) {
             const int n = t_end - t < 16 ? (int)(t_end - t) : 16;
             for (int i = 0; i < n; ++i)
-              expect(lds.tag[(size_t)(at + i)] == tag_of(t + i), "J byte from the staging byte of its time", t + i,
-                     lds.tag[(size_t)(at + i)] & 0xffffffffll);
+              expect(lds.tag[(size_t)(at + i)] == tag_of(srow, t + i), "J byte from the staging byte of its time", t + i,
+                     lds.tag[(size_t)(at + i)] & 0xffffffll);
             piece16(j_row_bfpt(b, F, f, srow & 1, T) + t, n);
           }
         } else {
@@ -200,8 +206,8 @@ void run_job(const Tiling &c, Buffers &buf, Staging &lds, int64_t e_base, int64_
           if (b < B && t < t_end) {
             const int n = t_end - t < 8 ? (int)(2 * (t_end - t)) : 16;
             for (int i = 0; i < n; ++i)
-              expect(lds.tag[(size_t)(at + i)] == tag_of(2 * t + i), "J byte from the staging byte of its time", 2 * t + i,
-                     lds.tag[(size_t)(at + i)] & 0xffffffffll);
+              expect(lds.tag[(size_t)(at + i)] == tag_of(srow, 2 * t + i), "J byte from the staging byte of its time",
+                     2 * t + i, lds.tag[(size_t)(at + i)] & 0xffffffll);
             piece16(j_row_bftp(b, F, f, T) + 2 * t, n);
           }
         }

@@ -28,3 +28,17 @@ def test_kernel_accesses_stay_in_bounds(tmp_path):
     run = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert run.returncode == 0, run.stdout + run.stderr
     assert run.stdout.strip().endswith("ok")
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+@pytest.mark.parametrize("plant", [1, 2, 3])
+def test_planted_mistakes_are_caught(tmp_path, plant):
+    # the model's own planted mistakes: no row clamp at T - 1, a read of the
+    # other staging slot, staging writes one row off
+    exe = tmp_path / f"bounds_check_plant{plant}"
+    cmd = [HIPCC, "-std=c++17", "-O2", f"-DPLANT={plant}", "-I", CSRC, "-I", INCLUDE,
+           os.path.join(HERE, "bounds_check.cpp"), "-o", str(exe)]
+    build = subprocess.run(cmd, capture_output=True, text=True)
+    assert build.returncode == 0, build.stderr
+    run = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert run.returncode != 0 and "FAIL" in run.stdout, run.stdout[-2000:]
